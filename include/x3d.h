@@ -1,0 +1,241 @@
+/*
+ * x3d.h -- C-ABI of the MI355X (gfx950) implementation of the Xcompact3d
+ * per-timestep hot path: compact finite-difference derivatives, staggered
+ * interpolations/derivatives, the spectral pressure Poisson solver and the
+ * pencil transposes.
+ *
+ * Conventions
+ *  - Every field argument is a DEVICE pointer to a Fortran-ordered array
+ *    (i fastest): element (i,j,k) (1-based) lives at (i-1)+n1*((j-1)+n2*(k-1)).
+ *    Callers whose arrays live in host memory use the x3d_host_* entries.
+ *  - Scalar arguments of the Fortran-mirror entries are passed by reference,
+ *    as a Fortran caller passes them (bind(C) without VALUE).
+ *  - The coefficient arrays (ff,fs,fw ...) are HOST pointers, exactly the
+ *    module arrays the reference callers pass; the library matches them to
+ *    the device copies it built in x3d_init (or uploads them if they are not
+ *    one of those sets).
+ *  - The scratch arguments r and s of the reference (rx, sx) are accepted and
+ *    ignored: the result lives only in t, as in the reference.
+ *  - Every entry returns 0 on success and a non-zero x3d_status otherwise;
+ *    x3d_last_error() describes the last failure of the calling thread.
+ *  - All work is enqueued on the library stream (x3d_set_stream); the
+ *    Fortran-mirror entries synchronise that stream before returning so that
+ *    their semantics equal the reference subroutine call; the *_async entries
+ *    do not.
+ *  - The library is not re-entrant: one context per process, as the
+ *    reference's module state (module_param.f90, variables.f90).
+ */
+#ifndef X3D_H
+#define X3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum x3d_status {
+  X3D_OK = 0,
+  X3D_ERR_ARG = 1,        /* bad argument (size, bc, pointer) */
+  X3D_ERR_STATE = 2,      /* x3d_init not called, or called twice */
+  X3D_ERR_HIP = 3,        /* a HIP runtime call failed */
+  X3D_ERR_UNSUPPORTED = 4 /* option the library does not implement */
+};
+
+/* Subset of the reference's namelist parameters that the hot path reads
+ * (parameters.f90:44-86; derived sizes parameters.f90:272-304). */
+typedef struct x3d_params {
+  int nx, ny, nz;                  /* velocity-grid points */
+  double xlx, yly, zlz;            /* domain lengths */
+  int nclx1, nclxn, ncly1, nclyn, nclz1, nclzn; /* 0 periodic, 1 free-slip, 2 Dirichlet */
+  int ifirstder;                   /* 4 = 6th-order compact (1 = 2nd-order central) */
+  int isecondder;                  /* 4 = 6th-order compact, 5 = hyperviscous */
+  int ipinter;                     /* 1,2,3: mid-point interpolation family (default 3) */
+  double re;                       /* xnu = 1/re */
+  double dt;
+  int itimescheme;                 /* 3 = AB3 (the only scheme of the step driver) */
+  int istret;                      /* 0 only */
+  double nu0nu, cnu;               /* isecondder=5 parameters */
+} x3d_params;
+
+/* ---- context -------------------------------------------------------------
+ * x3d_init replaces schemes() (schemes.f90:7-409) and
+ * decomp_2d_poisson_init() (poisson.f90:73-255): it computes every
+ * coefficient set in FP64 on the host, uploads FP64 and FP32 copies, and
+ * builds the modified-wavenumber table of waves() (poisson.f90:1530-1810). */
+int x3d_init(const x3d_params *p);
+int x3d_finalize(void);
+const char *x3d_last_error(void);
+int x3d_set_stream(void *hip_stream);      /* NULL = default stream */
+int x3d_synchronize(void);
+int x3d_device_name(char *buf, int len);
+
+/* Copy a coefficient array computed by x3d_init into host memory, so that a
+ * caller can hand the same arrays back (the reference's module arrays).
+ * name is the reference variable name: "ffx","fsx","fwx","ffxp",...,
+ * "sfx",...,"cfx6","csx6","cwx6","cifxp6",... ("y"/"z" variants likewise).
+ * Returns the array length in *n. */
+int x3d_get_coefficients(const char *name, double *out, int cap, int *n);
+/* Scalar coefficient by reference name: "afix","bfix","alfaix","asix",...,
+ * "acix6","alcaix6","aicix6","ailcaix6", "xnu", "dx", ... */
+int x3d_get_scalar(const char *name, double *out);
+/* Host-only evaluation of schemes() for the parameters p (no device, no
+ * context): an array or scalar by the same names as above. */
+int x3d_schemes_host(const x3d_params *p, const char *name, double *out, int cap, int *n);
+
+/* ---- compact derivatives -------------------------------------------------
+ * Replaces the procedure pointers derx/dery/derz/derxx/deryy/derzz bound in
+ * schemes.f90:31-66 with the abstract interfaces DERIVATIVE_X/Y/YY/Z of
+ * module_param.f90:135-168; one entry per BC variant of derive.f90:
+ *   derx_00 derive.f90:7    derx_11 :68    derx_12 :141   derx_21 :211   derx_22 :281
+ *   dery_00 :325            dery_11 :423   dery_12 :545   dery_21 :664   dery_22 :783
+ *   derz_00 :858            derz_11 :951   derz_12 :1064  derz_21 :1176  derz_22 :1288
+ *   derxx_00 :1354 derxx_11 :1481 derxx_12 :1666 derxx_21 :1822 derxx_22 :1978
+ *   deryy_00 :2052 deryy_11 :2207 deryy_12 :2433 deryy_21 :2631 deryy_22 :2829
+ *   derzz_00 :2923 derzz_11 :3082 derzz_12 :3307 derzz_21 :3503 derzz_22 :3699
+ * t,u device; r,s ignored; ff,fs,fw host; pp (y, first derivative only) host. */
+#define X3D_DECL_DER_X(name, T)                                                  \
+  int name(T *t, const T *u, T *r, T *s, const T *ff, const T *fs, const T *fw,  \
+           const int *nx, const int *ny, const int *nz, const int *npaire,       \
+           const T *lind);
+#define X3D_DECL_DER_Y(name, T)                                                  \
+  int name(T *t, const T *u, T *r, T *s, const T *ff, const T *fs, const T *fw,  \
+           const T *pp, const int *nx, const int *ny, const int *nz,             \
+           const int *npaire, const T *lind);
+
+#define X3D_DECL_DER_FAMILY(T, SFX)                                              \
+  X3D_DECL_DER_X(x3d_derx_00##SFX, T) X3D_DECL_DER_X(x3d_derx_11##SFX, T)        \
+  X3D_DECL_DER_X(x3d_derx_12##SFX, T) X3D_DECL_DER_X(x3d_derx_21##SFX, T)        \
+  X3D_DECL_DER_X(x3d_derx_22##SFX, T)                                            \
+  X3D_DECL_DER_Y(x3d_dery_00##SFX, T) X3D_DECL_DER_Y(x3d_dery_11##SFX, T)        \
+  X3D_DECL_DER_Y(x3d_dery_12##SFX, T) X3D_DECL_DER_Y(x3d_dery_21##SFX, T)        \
+  X3D_DECL_DER_Y(x3d_dery_22##SFX, T)                                            \
+  X3D_DECL_DER_X(x3d_derz_00##SFX, T) X3D_DECL_DER_X(x3d_derz_11##SFX, T)        \
+  X3D_DECL_DER_X(x3d_derz_12##SFX, T) X3D_DECL_DER_X(x3d_derz_21##SFX, T)        \
+  X3D_DECL_DER_X(x3d_derz_22##SFX, T)                                            \
+  X3D_DECL_DER_X(x3d_derxx_00##SFX, T) X3D_DECL_DER_X(x3d_derxx_11##SFX, T)      \
+  X3D_DECL_DER_X(x3d_derxx_12##SFX, T) X3D_DECL_DER_X(x3d_derxx_21##SFX, T)      \
+  X3D_DECL_DER_X(x3d_derxx_22##SFX, T)                                           \
+  X3D_DECL_DER_X(x3d_deryy_00##SFX, T) X3D_DECL_DER_X(x3d_deryy_11##SFX, T)      \
+  X3D_DECL_DER_X(x3d_deryy_12##SFX, T) X3D_DECL_DER_X(x3d_deryy_21##SFX, T)      \
+  X3D_DECL_DER_X(x3d_deryy_22##SFX, T)                                           \
+  X3D_DECL_DER_X(x3d_derzz_00##SFX, T) X3D_DECL_DER_X(x3d_derzz_11##SFX, T)      \
+  X3D_DECL_DER_X(x3d_derzz_12##SFX, T) X3D_DECL_DER_X(x3d_derzz_21##SFX, T)      \
+  X3D_DECL_DER_X(x3d_derzz_22##SFX, T)
+
+X3D_DECL_DER_FAMILY(double, )
+X3D_DECL_DER_FAMILY(float, _f32)
+
+/* ---- staggered operators (velocity grid n <-> pressure grid nm) ----------
+ * Same argument order as the reference external subroutines:
+ *   derxvp   derive.f90:3796  (tx,ux,rx,sx,cfx6,csx6,cwx6,nx,nxm,ny,nz,npaire)
+ *   interxvp derive.f90:3911  (tx,ux,rx,sx,cifx6,cisx6,ciwx6,nx,nxm,ny,nz,npaire)
+ *   derxpv   derive.f90:4041  (tx,ux,rx,sx,cfi6,csi6,cwi6,cfx6,csx6,cwx6,nxm,nx,ny,nz,npaire)
+ *   interxpv derive.f90:4126  (tx,ux,rx,sx,cifi6,cisi6,ciwi6,cifx6,cisx6,ciwx6,nxm,nx,ny,nz,npaire)
+ *   interyvp :4265 deryvp :4442 (+ppyi) interypv :4587 derypv :4775 (+ppy)
+ *   derzvp :4920 interzvp :5105 derzpv :5287 interzpv :5426
+ */
+#define X3D_DECL_STAG(T, SFX)                                                    \
+  int x3d_derxvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,    \
+                      const T *cw, const int *nx, const int *nxm, const int *ny, \
+                      const int *nz, const int *npaire);                         \
+  int x3d_interxvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,  \
+                        const T *cw, const int *nx, const int *nxm,              \
+                        const int *ny, const int *nz, const int *npaire);        \
+  int x3d_derxpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1,  \
+                      const T *cw1, const T *cf2, const T *cs2, const T *cw2,    \
+                      const int *nxm, const int *nx, const int *ny,              \
+                      const int *nz, const int *npaire);                         \
+  int x3d_interxpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1,              \
+                        const T *cs1, const T *cw1, const T *cf2, const T *cs2,  \
+                        const T *cw2, const int *nxm, const int *nx,             \
+                        const int *ny, const int *nz, const int *npaire);        \
+  int x3d_interyvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,  \
+                        const T *cw, const int *nx, const int *ny,               \
+                        const int *nym, const int *nz, const int *npaire);       \
+  int x3d_deryvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,    \
+                      const T *cw, const T *ppyi, const int *nx, const int *ny,  \
+                      const int *nym, const int *nz, const int *npaire);         \
+  int x3d_interypv##SFX(T *t, const T *u, T *r, T *s, const T *cf1,              \
+                        const T *cs1, const T *cw1, const T *cf2, const T *cs2,  \
+                        const T *cw2, const int *nx, const int *nym,             \
+                        const int *ny, const int *nz, const int *npaire);        \
+  int x3d_derypv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1,  \
+                      const T *cw1, const T *cf2, const T *cs2, const T *cw2,    \
+                      const T *ppy, const int *nx, const int *nym,               \
+                      const int *ny, const int *nz, const int *npaire);          \
+  int x3d_derzvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,    \
+                      const T *cw, const int *nx, const int *ny, const int *nz,  \
+                      const int *nzm, const int *npaire);                        \
+  int x3d_interzvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,  \
+                        const T *cw, const int *nx, const int *ny,               \
+                        const int *nz, const int *nzm, const int *npaire);       \
+  int x3d_derzpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1,  \
+                      const T *cw1, const T *cf2, const T *cs2, const T *cw2,    \
+                      const int *nx, const int *ny, const int *nzm,              \
+                      const int *nz, const int *npaire);                         \
+  int x3d_interzpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1,              \
+                        const T *cs1, const T *cw1, const T *cf2, const T *cs2,  \
+                        const T *cw2, const int *nx, const int *ny,              \
+                        const int *nzm, const int *nz, const int *npaire);
+
+X3D_DECL_STAG(double, )
+X3D_DECL_STAG(float, _f32)
+
+/* ---- pressure Poisson -----------------------------------------------------
+ * Replaces the procedure pointer `poisson` (poisson.f90:57-65) bound by
+ * decomp_2d_poisson_init to poisson_000 (poisson.f90:298) or poisson_010
+ * (poisson.f90:665).  rhs is the device pressure-grid field (nxm,nym,nzm),
+ * overwritten with the solution, exactly like poisson_xxx(rhs).  The _f32
+ * entry keeps FP32 storage and FFT arithmetic and divides by the modified
+ * wavenumber in FP64. */
+int x3d_poisson(double *rhs, const int *n1, const int *n2, const int *n3);
+int x3d_poisson_f32(float *rhs, const int *n1, const int *n2, const int *n3);
+
+/* ---- pencil transposes ----------------------------------------------------
+ * 2DECOMP&FFT v2.0.4 transpose_x_to_y / y_to_z / z_to_y / y_to_x (external;
+ * call sites e.g. transeq.f90:163-165, navier.f90:316-318).  On one GPU every
+ * pencil is the whole (n1,n2,n3) Fortran-ordered array, so each transpose is
+ * a device-resident copy of n1*n2*n3 words. */
+int x3d_transpose_x_to_y(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
+int x3d_transpose_y_to_z(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
+int x3d_transpose_z_to_y(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
+int x3d_transpose_y_to_x(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
+/* Explicit pencil reorder (i,j,k) -> permuted storage, LDS-tiled: perm is
+ * 0 = (j,i,k) [y-fastest], 1 = (k,j,i)... see x3d_reorder() documentation in
+ * the package; used when a pencil is stored with its own axis fastest. */
+int x3d_reorder_ji(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
+
+/* ---- host-memory entries (arrays in host memory; staged through HBM) ---- */
+int x3d_host_derx_00(double *t, const double *u, double *r, double *s, const double *ff,
+                     const double *fs, const double *fw, const int *nx, const int *ny,
+                     const int *nz, const int *npaire, const double *lind);
+int x3d_host_poisson(double *rhs, const int *n1, const int *n2, const int *n3);
+
+/* ---- device-resident time step (navier.f90 / transeq.f90 / time_integrators.f90)
+ * A solver owns the velocity (ux,uy,uz), the AB3 history (dux1..3 of
+ * time_integrators.f90:85-100) and all work arrays in HBM.
+ * precision: 0 = FP64 everywhere; 1 = FP32 bulk (FP32 derivatives, FFTs,
+ * convective products) with FP64 velocity accumulation, FP64 Poisson divide
+ * and FP64 diagnostics reductions. */
+typedef struct x3d_solver x3d_solver;
+int x3d_solver_create(int precision, x3d_solver **out);
+int x3d_solver_destroy(x3d_solver *s);
+/* Taylor-Green initial condition of Case-TGV.f90:64-96 (iin=1). */
+int x3d_solver_init_tgv(x3d_solver *s);
+/* Set the velocity from device FP64 arrays (nx,ny,nz) and reset itime. */
+int x3d_solver_set_velocity(x3d_solver *s, const double *ux, const double *uy, const double *uz);
+/* Copy the FP64 velocity to device arrays. */
+int x3d_solver_get_velocity(x3d_solver *s, double *ux, double *uy, double *uz);
+/* nsteps iterations of xcompact3d.f90:29-102 (transeq RHS, AB3, divergence,
+ * Poisson, gradp, cor_vel). */
+int x3d_solver_step(x3d_solver *s, int nsteps);
+/* (eek, eps, eps2, enst) of postprocess_tgv (Case-TGV.f90:246-377), FP64. */
+int x3d_solver_tgv_stats(x3d_solver *s, double out4[4]);
+/* Max and mean |div u| on the pressure grid after the projection
+ * (navier.f90:346-361 with nlock=2 semantics, FP64). */
+int x3d_solver_divergence_stats(x3d_solver *s, double out2[2]);
+int x3d_solver_itime(x3d_solver *s, int *itime);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* X3D_H */

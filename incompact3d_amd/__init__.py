@@ -1,0 +1,161 @@
+"""MI355X (gfx950) implementation of the Xcompact3d per-timestep hot path.
+
+The compute lives in ``libx3d.so`` (HIP, C-ABI in ``include/x3d.h``); this
+package is the host-side mirror of the reference interface:
+
+* :func:`init` / :func:`finalize`  -- schemes() + decomp_2d_poisson_init()
+* :data:`coef`                      -- the module coefficient arrays/scalars
+* :mod:`.derive`                    -- derx_00 ... interzpv (derive.f90)
+* :func:`poisson`                   -- poisson_000 / poisson_010 (poisson.f90)
+* :func:`transpose_x_to_y` ...      -- 2DECOMP pencil transposes
+* :class:`Solver`                   -- the device-resident time step
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Params, X3DError, check, iref, make_params
+from . import derive  # noqa: F401
+
+__all__ = ["Params", "X3DError", "make_params", "init", "finalize", "coef", "poisson",
+           "transpose_x_to_y", "transpose_y_to_z", "transpose_z_to_y", "transpose_y_to_x",
+           "reorder_ji", "Solver", "derive", "device_name", "schemes_host"]
+
+
+def init(p: Params) -> None:
+    """x3d_init: schemes() (schemes.f90:7) and decomp_2d_poisson_init() (poisson.f90:73)."""
+    check(_lib.lib().x3d_init(C.byref(p)), "x3d_init")
+
+
+def finalize() -> None:
+    check(_lib.lib().x3d_finalize(), "x3d_finalize")
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(_lib.lib().x3d_device_name(buf, 256), "x3d_device_name")
+    return buf.value.decode()
+
+
+class _Coef:
+    """Attribute access to the coefficient state of the initialised context,
+    by reference variable name: ``coef.ffx``, ``coef.cifxp6``, ``coef.afix``."""
+
+    def __getattr__(self, name):
+        lib = _lib.lib()
+        n = C.c_int(0)
+        if lib.x3d_get_coefficients(name.encode(), None, 0, C.byref(n)) == 0:
+            out = np.empty(n.value, dtype=np.float64)
+            check(lib.x3d_get_coefficients(name.encode(), out.ctypes.data_as(C.POINTER(C.c_double)),
+                                           n.value, C.byref(n)), name)
+            return out
+        v = C.c_double(0.0)
+        check(lib.x3d_get_scalar(name.encode(), C.byref(v)), name)
+        return v.value
+
+
+coef = _Coef()
+
+
+def schemes_host(p: Params, name: str):
+    """Host-only evaluation of one coefficient (array or scalar) for ``p``."""
+    lib = _lib.lib()
+    n = C.c_int(0)
+    check(lib.x3d_schemes_host(C.byref(p), name.encode(), None, 0, C.byref(n)), name)
+    out = np.empty(max(n.value, 1), dtype=np.float64)
+    check(lib.x3d_schemes_host(C.byref(p), name.encode(), out.ctypes.data_as(C.POINTER(C.c_double)),
+                               out.size, C.byref(n)), name)
+    return out if n.value > 1 else float(out[0])
+
+
+def _dims(t):
+    nz, ny, nx = t.shape
+    return nx, ny, nz
+
+
+def poisson(rhs) -> None:
+    """poisson(rhs): the procedure pointer of poisson.f90:57-65 (in place).
+    rhs is a device tensor on the pressure grid, shape (nzm, nym, nxm)."""
+    import torch
+    n1, n2, n3 = _dims(rhs)
+    fn = _lib.lib().x3d_poisson if rhs.dtype == torch.float64 else _lib.lib().x3d_poisson_f32
+    check(fn(derive._dptr(rhs), iref(n1), iref(n2), iref(n3)), "poisson")
+
+
+def _transpose(name):
+    def f(src, dst):
+        n1, n2, n3 = _dims(src)
+        check(getattr(_lib.lib(), f"x3d_{name}")(derive._dptr(src), derive._dptr(dst), iref(n1),
+                                                  iref(n2), iref(n3)), name)
+    f.__name__ = name
+    return f
+
+
+transpose_x_to_y = _transpose("transpose_x_to_y")
+transpose_y_to_z = _transpose("transpose_y_to_z")
+transpose_z_to_y = _transpose("transpose_z_to_y")
+transpose_y_to_x = _transpose("transpose_y_to_x")
+reorder_ji = _transpose("reorder_ji")
+
+
+class Solver:
+    """Device-resident TGV/periodic solver (x3d_solver_*).
+
+    precision 0: FP64 everywhere.  precision 1: FP32 bulk with FP64 velocity
+    accumulation, FP64 Poisson divide and FP64 reductions."""
+
+    def __init__(self, p: Params, precision: int = 0):
+        self._h = C.c_void_p()
+        check(_lib.lib().x3d_solver_create(int(precision), C.byref(self._h)), "x3d_solver_create")
+        self.precision = precision
+        self._shape = (p.nz, p.ny, p.nx)
+
+    def close(self):
+        if self._h:
+            _lib.lib().x3d_solver_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_tgv(self):
+        check(_lib.lib().x3d_solver_init_tgv(self._h), "init_tgv")
+
+    def set_velocity(self, ux, uy, uz):
+        check(_lib.lib().x3d_solver_set_velocity(self._h, derive._dptr(ux), derive._dptr(uy),
+                                                 derive._dptr(uz)), "set_velocity")
+
+    def velocity(self):
+        """FP64 copies (ux, uy, uz) of the solver velocity, device tensors."""
+        import torch
+        out = [torch.empty(self._shape, dtype=torch.float64, device="cuda") for _ in range(3)]
+        check(_lib.lib().x3d_solver_get_velocity(self._h, *[derive._dptr(t) for t in out]),
+              "get_velocity")
+        return out
+
+    def step(self, nsteps: int = 1):
+        check(_lib.lib().x3d_solver_step(self._h, int(nsteps)), "step")
+
+    def tgv_stats(self):
+        """(eek, eps, eps2, enst) of postprocess_tgv."""
+        out = (C.c_double * 4)()
+        check(_lib.lib().x3d_solver_tgv_stats(self._h, out), "tgv_stats")
+        return tuple(out)
+
+    def divergence_stats(self):
+        out = (C.c_double * 2)()
+        check(_lib.lib().x3d_solver_divergence_stats(self._h, out), "divergence_stats")
+        return tuple(out)
+
+    @property
+    def itime(self):
+        v = C.c_int(0)
+        check(_lib.lib().x3d_solver_itime(self._h, C.byref(v)), "itime")
+        return v.value
+

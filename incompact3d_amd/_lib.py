@@ -1,0 +1,108 @@
+"""ctypes binding of libx3d.so (the C-ABI declared in include/x3d.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when ``libx3d.so`` has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libx3d.so")
+
+
+class X3DError(RuntimeError):
+    """A non-zero status returned by the C-ABI (message from x3d_last_error)."""
+
+
+class Params(C.Structure):
+    """x3d_params: the namelist subset of parameters.f90 that the hot path reads."""
+
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("xlx", C.c_double), ("yly", C.c_double), ("zlz", C.c_double),
+        ("nclx1", C.c_int), ("nclxn", C.c_int), ("ncly1", C.c_int), ("nclyn", C.c_int),
+        ("nclz1", C.c_int), ("nclzn", C.c_int),
+        ("ifirstder", C.c_int), ("isecondder", C.c_int), ("ipinter", C.c_int),
+        ("re", C.c_double), ("dt", C.c_double), ("itimescheme", C.c_int), ("istret", C.c_int),
+        ("nu0nu", C.c_double), ("cnu", C.c_double),
+    ]
+
+
+def make_params(nx, ny, nz, xlx=None, yly=None, zlz=None, nclx1=0, nclxn=0, ncly1=0, nclyn=0,
+                nclz1=0, nclzn=0, ifirstder=4, isecondder=4, ipinter=3, re=1600.0, dt=0.005,
+                itimescheme=3, istret=0, nu0nu=4.0, cnu=0.44) -> Params:
+    import math
+    twopi = 2.0 * math.acos(-1.0)
+    return Params(nx, ny, nz, twopi if xlx is None else xlx, twopi if yly is None else yly,
+                  twopi if zlz is None else zlz, nclx1, nclxn, ncly1, nclyn, nclz1, nclzn,
+                  ifirstder, isecondder, ipinter, float(re), float(dt), itimescheme, istret,
+                  float(nu0nu), float(cnu))
+
+
+_lib = None
+
+P = C.c_void_p
+IP = C.POINTER(C.c_int)
+
+
+def _declare(lib):
+    lib.x3d_last_error.restype = C.c_char_p
+    lib.x3d_last_error.argtypes = []
+    lib.x3d_init.argtypes = [C.POINTER(Params)]
+    for name in ("x3d_finalize", "x3d_synchronize"):
+        getattr(lib, name).argtypes = []
+    lib.x3d_set_stream.argtypes = [P]
+    lib.x3d_device_name.argtypes = [C.c_char_p, C.c_int]
+    lib.x3d_get_coefficients.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int, IP]
+    lib.x3d_get_scalar.argtypes = [C.c_char_p, C.POINTER(C.c_double)]
+    lib.x3d_schemes_host.argtypes = [C.POINTER(Params), C.c_char_p, C.POINTER(C.c_double), C.c_int, IP]
+    for sfx in ("", "_f32"):
+        for d in ("x", "z", "xx", "yy", "zz"):
+            for bc in ("00", "11", "12", "21", "22"):
+                getattr(lib, f"x3d_der{d}_{bc}{sfx}").argtypes = [P, P, P, P, P, P, P, IP, IP, IP, IP, P]
+        for bc in ("00", "11", "12", "21", "22"):
+            getattr(lib, f"x3d_dery_{bc}{sfx}").argtypes = [P, P, P, P, P, P, P, P, IP, IP, IP, IP, P]
+        for nm in ("derxvp", "interxvp", "interyvp", "derzvp", "interzvp"):
+            getattr(lib, f"x3d_{nm}{sfx}").argtypes = [P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
+        getattr(lib, f"x3d_deryvp{sfx}").argtypes = [P, P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
+        for nm in ("derxpv", "interxpv", "interypv", "derzpv", "interzpv"):
+            getattr(lib, f"x3d_{nm}{sfx}").argtypes = [P, P, P, P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
+        getattr(lib, f"x3d_derypv{sfx}").argtypes = [P, P, P, P, P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
+        getattr(lib, f"x3d_poisson{sfx}").argtypes = [P, IP, IP, IP]
+    for nm in ("x3d_transpose_x_to_y", "x3d_transpose_y_to_z", "x3d_transpose_z_to_y",
+               "x3d_transpose_y_to_x", "x3d_reorder_ji"):
+        getattr(lib, nm).argtypes = [P, P, IP, IP, IP]
+    lib.x3d_host_derx_00.argtypes = [P, P, P, P, P, P, P, IP, IP, IP, IP, P]
+    lib.x3d_host_poisson.argtypes = [P, IP, IP, IP]
+    lib.x3d_solver_create.argtypes = [C.c_int, C.POINTER(P)]
+    lib.x3d_solver_destroy.argtypes = [P]
+    lib.x3d_solver_init_tgv.argtypes = [P]
+    lib.x3d_solver_set_velocity.argtypes = [P, P, P, P]
+    lib.x3d_solver_get_velocity.argtypes = [P, P, P, P]
+    lib.x3d_solver_step.argtypes = [P, C.c_int]
+    lib.x3d_solver_tgv_stats.argtypes = [P, C.POINTER(C.c_double)]
+    lib.x3d_solver_divergence_stats.argtypes = [P, C.POINTER(C.c_double)]
+    lib.x3d_solver_itime.argtypes = [P, IP]
+
+
+def lib():
+    """Load libx3d.so (once).  Raises if the library has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise X3DError(f"{LIB_PATH} is missing: build it with `python -m incompact3d_amd.build`")
+        _lib = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        _declare(_lib)
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        msg = lib().x3d_last_error().decode(errors="replace")
+        raise X3DError(f"{what} failed (status {status}): {msg}")
+
+
+def iref(v: int):
+    return C.byref(C.c_int(int(v)))

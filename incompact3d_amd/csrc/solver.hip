@@ -1,0 +1,385 @@
+// Device-resident time step of the incompressible solver (periodic box):
+//   calculate_transeq_rhs  transeq.f90:20-591   (skew-symmetric convection + nu lap)
+//   int_time (AB3)         time_integrators.f90:85-100
+//   solve_poisson          navier.f90:27-141    (divergence, poisson, gradp)
+//   cor_vel                navier.f90:206-249
+// as in the time loop of xcompact3d.f90:29-102.  Every derivative and
+// staggered operator is one line-solve launch whose epilogue does the
+// reference's array arithmetic (products, sums, the pressure correction), so
+// no separate elementwise pass touches HBM.  precision 0 computes in FP64;
+// precision 1 keeps an FP32 copy of the velocity for every derivative, FFT
+// and product, while the velocity itself is accumulated in FP64 (AB3 and
+// cor_vel) and the Poisson divide and reductions run in FP64.
+#include <cmath>
+#include <cstring>
+
+#include "x3d_internal.h"
+
+namespace x3d {
+
+template <typename H>
+int launch_ab(double *u, const H *f1, const H *f2, const H *f3, double a, double b, double c, int nterms,
+              float *shadow, size_t n, hipStream_t st);
+template <typename P>
+int launch_corvel(double *u, const P *p, float *shadow, size_t n, hipStream_t st);
+int launch_cast(const double *a, float *b, size_t n, hipStream_t st);
+int launch_tgv(double *ux, double *uy, double *uz, int nx, int ny, int nz, double dx, double dy, double dz,
+               hipStream_t st);
+template <typename T>
+int launch_tgv_sums(const T *ux, const T *uy, const T *uz, const T *const g[9], const T *lx, const T *ly,
+                    const T *lz, double xnu, size_t n, double *partial, double *out4, hipStream_t st);
+template <typename T>
+int launch_max_mean(const T *a, size_t n, double *partial, double *out2, hipStream_t st);
+
+}  // namespace x3d
+
+struct x3d_solver {
+  int precision = 0;
+  int itime = 0;
+  size_t n = 0;
+  int nx = 0, ny = 0, nz = 0;
+  double *u64[3] = {nullptr, nullptr, nullptr};
+  float *u32[3] = {nullptr, nullptr, nullptr};
+  void *hist[3][3] = {};       // [component][slot], compute precision
+  int slot[3] = {0, 1, 2};     // slot index of f1, f2, f3
+  void *w[4] = {};             // work fields, compute precision
+  void *diag = nullptr;        // 12 fields for the TGV statistics
+  double *w64[4] = {};         // FP64 work for the divergence check of precision 1
+  double *partial = nullptr;   // reduction partials
+  double *red = nullptr;       // reduction results (device)
+};
+
+namespace x3d {
+namespace {
+
+template <typename T>
+int line(const Context &c, int dir, int op, int npaire, const T *u, const T *u2, T *t, const T *acc, const T *mul,
+         double ac, double rc, int nx, int ny, int nz) {
+  LineCall<T> L;
+  X3D_TRY(make_line_call<T>(c, dir, op, npaire, &L));
+  L.nx = nx;
+  L.ny = ny;
+  L.nz = nz;
+  L.u = u;
+  L.u2 = u2;
+  L.t = t;
+  L.acc = acc;
+  L.mul = mul;
+  L.ac = (T)ac;
+  L.rc = (T)rc;
+  return launch_line<T>(L, c.stream);
+}
+
+// momentum_rhs_eq (transeq.f90:73-470) for one component c of the velocity:
+//   out = xnu*(d2x+d2y+d2z) u_c - 1/2 sum_d [ d_d(u_d u_c) + u_d d_d u_c ]
+// accumulated in the reference's order z, y, x.  Parities follow the
+// reference calls: the product and velocity derivative along d use the even
+// (p) coefficient sets when d == c for the product, when d != c otherwise.
+template <typename T>
+int momentum_component(const Context &c, const T *const u[3], int comp, T *W, T *out, int nx, int ny, int nz) {
+  const double xnu = c.xnu;
+  const T *uc = u[comp];
+  for (int d = 2; d >= 0; --d) {
+    const int pp = d == comp ? 1 : 0;   // parity of the product u_d*u_c
+    const int pv = d == comp ? 0 : 1;   // parity of u_c along d
+    if (d == 2) {
+      X3D_TRY(line<T>(c, d, OP_DER1, pp, u[d], uc, W, nullptr, nullptr, 0.0, 1.0, nx, ny, nz));
+      X3D_TRY(line<T>(c, d, OP_DER1, pv, uc, nullptr, W, W, u[d], 1.0, 1.0, nx, ny, nz));
+      X3D_TRY(line<T>(c, d, OP_DER2, pv, uc, nullptr, W, W, nullptr, -0.5, xnu, nx, ny, nz));
+    } else {
+      X3D_TRY(line<T>(c, d, OP_DER1, pp, u[d], uc, W, W, nullptr, 1.0, -0.5, nx, ny, nz));
+      X3D_TRY(line<T>(c, d, OP_DER1, pv, uc, nullptr, W, W, u[d], 1.0, -0.5, nx, ny, nz));
+      T *dst = d == 0 ? out : W;
+      X3D_TRY(line<T>(c, d, OP_DER2, pv, uc, nullptr, dst, W, nullptr, 1.0, xnu, nx, ny, nz));
+    }
+  }
+  return X3D_OK;
+}
+
+// divergence (navier.f90:257-372), result (pressure grid) in W[0]
+template <typename T>
+int divergence(const Context &c, const T *const u[3], T *const W[4], int nx, int ny, int nz) {
+  const int nxm = c.nxm, nym = c.nym;
+  X3D_TRY(line<T>(c, 0, OP_DERVP, 0, u[0], nullptr, W[0], nullptr, nullptr, 0, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 0, OP_INTERVP, 1, u[1], nullptr, W[1], nullptr, nullptr, 0, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 0, OP_INTERVP, 1, u[2], nullptr, W[2], nullptr, nullptr, 0, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 1, OP_INTERVP, 1, W[0], nullptr, W[3], nullptr, nullptr, 0, 1, nxm, ny, nz));
+  X3D_TRY(line<T>(c, 1, OP_DERVP, 0, W[1], nullptr, W[3], W[3], nullptr, 1, 1, nxm, ny, nz));
+  X3D_TRY(line<T>(c, 1, OP_INTERVP, 1, W[2], nullptr, W[1], nullptr, nullptr, 0, 1, nxm, ny, nz));
+  X3D_TRY(line<T>(c, 2, OP_INTERVP, 1, W[3], nullptr, W[0], nullptr, nullptr, 0, 1, nxm, nym, nz));
+  X3D_TRY(line<T>(c, 2, OP_DERVP, 0, W[1], nullptr, W[0], W[0], nullptr, 1, 1, nxm, nym, nz));
+  return X3D_OK;
+}
+
+// gradp (navier.f90:386-431) with cor_vel fused: u_i <- u_i - grad_i p.
+// pp3 (pressure grid) in W[0]; for FP32 compute the gradient goes to W[2]
+// and is subtracted from the FP64 velocity.
+template <typename T>
+int gradp_correct(const Context &c, x3d_solver *S, T *const u[3], T *const W[4]) {
+  const int nx = S->nx, ny = S->ny, nz = S->nz, nxm = c.nxm, nym = c.nym, nzm = c.nzm;
+  X3D_TRY(line<T>(c, 2, OP_INTERPV, 1, W[0], nullptr, W[1], nullptr, nullptr, 0, 1, nxm, nym, nzm));
+  X3D_TRY(line<T>(c, 2, OP_DERPV, 1, W[0], nullptr, W[2], nullptr, nullptr, 0, 1, nxm, nym, nzm));
+  X3D_TRY(line<T>(c, 1, OP_INTERPV, 1, W[1], nullptr, W[3], nullptr, nullptr, 0, 1, nxm, nym, nz));
+  X3D_TRY(line<T>(c, 1, OP_DERPV, 1, W[1], nullptr, W[0], nullptr, nullptr, 0, 1, nxm, nym, nz));
+  X3D_TRY(line<T>(c, 1, OP_INTERPV, 1, W[2], nullptr, W[1], nullptr, nullptr, 0, 1, nxm, nym, nz));
+  // px from W[3], py from W[0], pz from W[1]
+  const T *src[3] = {W[3], W[0], W[1]};
+  const int ops[3] = {OP_DERPV, OP_INTERPV, OP_INTERPV};
+  for (int i = 0; i < 3; ++i) {
+    if constexpr (sizeof(T) == 8) {
+      X3D_TRY(line<T>(c, 0, ops[i], 1, src[i], nullptr, u[i], u[i], nullptr, 1.0, -1.0, nxm, ny, nz));
+    } else {
+      X3D_TRY(line<T>(c, 0, ops[i], 1, src[i], nullptr, W[2], nullptr, nullptr, 0, 1, nxm, ny, nz));
+      X3D_TRY(launch_corvel<float>(S->u64[i], (const float *)W[2], S->u32[i], S->n, c.stream));
+    }
+  }
+  return X3D_OK;
+}
+
+template <typename T>
+int step_once(Context &c, x3d_solver *S) {
+  const int nx = S->nx, ny = S->ny, nz = S->nz;
+  T *u[3];
+  for (int i = 0; i < 3; ++i) u[i] = sizeof(T) == 8 ? (T *)S->u64[i] : (T *)S->u32[i];
+  T *W[4] = {(T *)S->w[0], (T *)S->w[1], (T *)S->w[2], (T *)S->w[3]};
+  S->itime += 1;
+  // transeq: new RHS into the f1 slot of each component
+  const T *cu[3] = {u[0], u[1], u[2]};
+  for (int comp = 0; comp < 3; ++comp) {
+    T *f1 = (T *)S->hist[comp][S->slot[0]];
+    X3D_TRY(momentum_component<T>(c, cu, comp, W[0], f1, nx, ny, nz));
+  }
+  // AB3 (time_integrators.f90:85-100)
+  const double dt = c.p.dt;
+  double a, b = 0, cc = 0;
+  int nterms;
+  if (S->itime == 1) {
+    a = dt; nterms = 1;
+  } else if (S->itime == 2) {
+    a = 1.5 * dt; b = -(0.5 * dt); nterms = 2;
+  } else {
+    a = (23.0 / 12.0) * dt; b = -(16.0 / 12.0) * dt; cc = (5.0 / 12.0) * dt; nterms = 3;
+  }
+  for (int comp = 0; comp < 3; ++comp) {
+    const T *f1 = (T *)S->hist[comp][S->slot[0]];
+    const T *f2 = (T *)S->hist[comp][S->slot[1]];
+    const T *f3 = (T *)S->hist[comp][S->slot[2]];
+    X3D_TRY(launch_ab<T>(S->u64[comp], f1, f2, f3, a, b, cc, nterms, sizeof(T) == 8 ? nullptr : S->u32[comp],
+                         S->n, c.stream));
+  }
+  // history rotation (f3 <- f2 <- f1) by slot permutation
+  const int s1 = S->slot[0], s2 = S->slot[1], s3 = S->slot[2];
+  S->slot[0] = s3;
+  S->slot[1] = s1;
+  S->slot[2] = s2;
+  // projection
+  X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));
+  X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
+  X3D_TRY(gradp_correct<T>(c, S, u, W));
+  return X3D_OK;
+}
+
+template <typename T>
+int tgv_stats(Context &c, x3d_solver *S, double out4[4]) {
+  const int nx = S->nx, ny = S->ny, nz = S->nz;
+  const size_t n = S->n;
+  if (!S->diag) X3D_HIP(hipMalloc(&S->diag, 12 * n * sizeof(T)));
+  T *g[12];
+  for (int i = 0; i < 12; ++i) g[i] = (T *)S->diag + i * n;
+  const T *u[3];
+  for (int i = 0; i < 3; ++i) u[i] = sizeof(T) == 8 ? (const T *)S->u64[i] : (const T *)S->u32[i];
+  // (dux/dx, duy/dx, duz/dx, dux/dy, ..., duz/dz) with the parities of Case-TGV.f90:262-270
+  for (int d = 0; d < 3; ++d)
+    for (int comp = 0; comp < 3; ++comp) {
+      const int pv = d == comp ? 0 : 1;
+      X3D_TRY(line<T>(c, d, OP_DER1, pv, u[comp], nullptr, g[3 * d + comp], nullptr, nullptr, 0, 1, nx, ny, nz));
+    }
+  // laplacian per component: d2x + d2y + d2z (Case-TGV.f90:330-360)
+  for (int comp = 0; comp < 3; ++comp)
+    for (int d = 0; d < 3; ++d) {
+      const int pv = d == comp ? 0 : 1;
+      T *lap = g[9 + comp];
+      X3D_TRY(line<T>(c, d, OP_DER2, pv, u[comp], nullptr, lap, d == 0 ? nullptr : lap, nullptr, 1, 1, nx, ny, nz));
+    }
+  const T *gg[9];
+  for (int i = 0; i < 9; ++i) gg[i] = g[i];
+  if (!S->partial) X3D_HIP(hipMalloc(&S->partial, 8 * 1024 * sizeof(double)));
+  if (!S->red) X3D_HIP(hipMalloc(&S->red, 8 * sizeof(double)));
+  X3D_TRY(launch_tgv_sums<T>(u[0], u[1], u[2], gg, g[9], g[10], g[11], c.xnu, n, S->partial, S->red, c.stream));
+  double h[4];
+  X3D_HIP(hipMemcpyAsync(h, S->red, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  X3D_HIP(hipStreamSynchronize(c.stream));
+  const double ncell = (double)nx * ny * nz;
+  out4[0] = h[0] / ncell;   // eek
+  out4[1] = h[1] / ncell;   // eps
+  out4[2] = h[3] / ncell;   // eps2
+  out4[3] = h[2] / ncell;   // enst
+  return X3D_OK;
+}
+
+int alloc_solver(Context &c, x3d_solver *S) {
+  const size_t n = S->n;
+  const size_t es = S->precision == 0 ? 8 : 4;
+  for (int i = 0; i < 3; ++i) {
+    X3D_HIP(hipMalloc(&S->u64[i], n * 8));
+    if (S->precision == 1) X3D_HIP(hipMalloc(&S->u32[i], n * 4));
+    for (int j = 0; j < 3; ++j) {
+      X3D_HIP(hipMalloc(&S->hist[i][j], n * es));
+      X3D_HIP(hipMemsetAsync(S->hist[i][j], 0, n * es, c.stream));
+    }
+  }
+  const size_t np = (size_t)c.nxm * c.nym * c.nzm;
+  const size_t nw = n > np ? n : np;
+  for (int i = 0; i < 4; ++i) X3D_HIP(hipMalloc(&S->w[i], nw * es));
+  X3D_HIP(hipStreamSynchronize(c.stream));
+  return X3D_OK;
+}
+
+void free_solver(x3d_solver *S) {
+  for (int i = 0; i < 3; ++i) {
+    if (S->u64[i]) (void)hipFree(S->u64[i]);
+    if (S->u32[i]) (void)hipFree(S->u32[i]);
+    for (int j = 0; j < 3; ++j)
+      if (S->hist[i][j]) (void)hipFree(S->hist[i][j]);
+  }
+  for (int i = 0; i < 4; ++i) {
+    if (S->w[i]) (void)hipFree(S->w[i]);
+    if (S->w64[i]) (void)hipFree(S->w64[i]);
+  }
+  if (S->diag) (void)hipFree(S->diag);
+  if (S->partial) (void)hipFree(S->partial);
+  if (S->red) (void)hipFree(S->red);
+}
+
+int reset_history(Context &c, x3d_solver *S) {
+  // case.f90:131-136 seeds the history with u; step 1 (Euler) and step 2
+  // (AB2) never read the seeded values, so the slots only need defined data.
+  const size_t es = S->precision == 0 ? 8 : 4;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) X3D_HIP(hipMemsetAsync(S->hist[i][j], 0, S->n * es, c.stream));
+  S->slot[0] = 0;
+  S->slot[1] = 1;
+  S->slot[2] = 2;
+  S->itime = 0;
+  if (S->precision == 1)
+    for (int i = 0; i < 3; ++i) X3D_TRY(launch_cast(S->u64[i], S->u32[i], S->n, c.stream));
+  X3D_HIP(hipStreamSynchronize(c.stream));
+  return X3D_OK;
+}
+
+}  // namespace
+}  // namespace x3d
+
+using namespace x3d;
+
+extern "C" {
+
+int x3d_solver_create(int precision, x3d_solver **out) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!out) return fail(X3D_ERR_ARG, "null output");
+  if (precision != 0 && precision != 1) return fail(X3D_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 bulk)");
+  if (c->dir[0].bc1 != 0 || c->dir[1].bc1 != 0 || c->dir[2].bc1 != 0)
+    return fail(X3D_ERR_UNSUPPORTED, "the device time step covers triply periodic boxes");
+  if (!c->poisson) return fail(X3D_ERR_UNSUPPORTED, "no Poisson solver for this grid");
+  if (c->p.itimescheme != 3) return fail(X3D_ERR_UNSUPPORTED, "the device time step implements AB3 (itimescheme=3)");
+  auto *S = new x3d_solver();
+  S->precision = precision;
+  S->nx = c->p.nx;
+  S->ny = c->p.ny;
+  S->nz = c->p.nz;
+  S->n = (size_t)S->nx * S->ny * S->nz;
+  int r = alloc_solver(*c, S);
+  if (r != X3D_OK) {
+    free_solver(S);
+    delete S;
+    return r;
+  }
+  *out = S;
+  return X3D_OK;
+}
+
+int x3d_solver_destroy(x3d_solver *S) {
+  if (!S) return X3D_OK;
+  (void)hipDeviceSynchronize();
+  free_solver(S);
+  delete S;
+  return X3D_OK;
+}
+
+int x3d_solver_init_tgv(x3d_solver *S) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S) return fail(X3D_ERR_ARG, "null solver");
+  X3D_TRY(launch_tgv(S->u64[0], S->u64[1], S->u64[2], S->nx, S->ny, S->nz, c->dir[0].d, c->dir[1].d, c->dir[2].d,
+                     c->stream));
+  return reset_history(*c, S);
+}
+
+int x3d_solver_set_velocity(x3d_solver *S, const double *ux, const double *uy, const double *uz) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || !ux || !uy || !uz) return fail(X3D_ERR_ARG, "null pointer");
+  const double *src[3] = {ux, uy, uz};
+  for (int i = 0; i < 3; ++i)
+    X3D_HIP(hipMemcpyAsync(S->u64[i], src[i], S->n * 8, hipMemcpyDeviceToDevice, c->stream));
+  return reset_history(*c, S);
+}
+
+int x3d_solver_get_velocity(x3d_solver *S, double *ux, double *uy, double *uz) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || !ux || !uy || !uz) return fail(X3D_ERR_ARG, "null pointer");
+  double *dst[3] = {ux, uy, uz};
+  for (int i = 0; i < 3; ++i)
+    X3D_HIP(hipMemcpyAsync(dst[i], S->u64[i], S->n * 8, hipMemcpyDeviceToDevice, c->stream));
+  X3D_HIP(hipStreamSynchronize(c->stream));
+  return X3D_OK;
+}
+
+int x3d_solver_step(x3d_solver *S, int nsteps) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || nsteps < 0) return fail(X3D_ERR_ARG, "bad arguments");
+  for (int k = 0; k < nsteps; ++k) {
+    if (S->precision == 0) X3D_TRY(step_once<double>(*c, S));
+    else X3D_TRY(step_once<float>(*c, S));
+  }
+  return X3D_OK;
+}
+
+int x3d_solver_tgv_stats(x3d_solver *S, double out4[4]) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || !out4) return fail(X3D_ERR_ARG, "null pointer");
+  if (S->precision == 0) return tgv_stats<double>(*c, S, out4);
+  return tgv_stats<float>(*c, S, out4);
+}
+
+int x3d_solver_divergence_stats(x3d_solver *S, double out2[2]) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || !out2) return fail(X3D_ERR_ARG, "null pointer");
+  // FP64 divergence of the FP64 velocity, whatever the compute precision
+  const size_t np = (size_t)c->nxm * c->nym * c->nzm;
+  const size_t nw = S->n > np ? S->n : np;
+  for (int i = 0; i < 4; ++i)
+    if (!S->w64[i]) X3D_HIP(hipMalloc(&S->w64[i], nw * 8));
+  double *W[4] = {S->w64[0], S->w64[1], S->w64[2], S->w64[3]};
+  const double *u[3] = {S->u64[0], S->u64[1], S->u64[2]};
+  X3D_TRY(divergence<double>(*c, u, W, S->nx, S->ny, S->nz));
+  if (!S->partial) X3D_HIP(hipMalloc(&S->partial, 8 * 1024 * sizeof(double)));
+  if (!S->red) X3D_HIP(hipMalloc(&S->red, 8 * sizeof(double)));
+  X3D_TRY(launch_max_mean<double>(W[0], np, S->partial, S->red, c->stream));
+  X3D_HIP(hipMemcpyAsync(out2, S->red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  X3D_HIP(hipStreamSynchronize(c->stream));
+  return X3D_OK;
+}
+
+int x3d_solver_itime(x3d_solver *S, int *itime) {
+  if (!S || !itime) return fail(X3D_ERR_ARG, "null pointer");
+  *itime = S->itime;
+  return X3D_OK;
+}
+
+}  // extern "C"
