@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Throughput of the Xcompact3d hot path on one MI355X.
+
+Workload: Taylor-Green vortex, 256^3 triply periodic box [0,2pi)^3, Re=1600,
+6th-order compact schemes, spectral Poisson, AB3 with dt=0.002 -- one "step"
+is one full time step (momentum RHS, AB3, divergence, Poisson, gradient,
+correction).  Prints ONE JSON line; ``value`` is the FP64 path in
+Mpoint-steps/s (grid points x steps / s), the FP32-bulk path is reported
+beside it with its dissipation error against the FP64 run.
+
+    python bench.py [--steps K] [--warmup W] [--n 256]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+
+def _events_time(stream, fn, reps):
+    import torch
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record(stream)
+    for _ in range(reps):
+        fn()
+    e.record(stream)
+    e.synchronize()
+    return s.elapsed_time(e) / reps   # ms per launch
+
+
+def run_solver(X, p, precision, warmup, steps, stream):
+    import torch
+    s = X.Solver(p, precision)
+    s.init_tgv()
+    s.step(warmup)
+    torch.cuda.synchronize()
+    if torch.distributed.is_available() and torch.distributed.is_initialized():
+        torch.distributed.barrier()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    s.step(steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    if torch.distributed.is_available() and torch.distributed.is_initialized():
+        torch.distributed.barrier()
+    stats = s.tgv_stats()
+    div = s.divergence_stats()
+    s.close()
+    return {"wall_ms_per_step": (t1 - t0) * 1e3 / steps,
+            "event_ms_per_step": ev0.elapsed_time(ev1) / steps,
+            "stats": stats, "div": div}
+
+
+def roofline_line_kernel(X, n, stream, reps=40):
+    """Average duration of the dominant kernel -- the FP64 line-solve kernel
+    of a first derivative along y on the n^3 field -- measured with HIP events
+    on the library stream.  Algorithmic bytes per launch: 2*N*8 (one read of
+    u, one write of du/dy)."""
+    import torch
+    u = torch.randn((n, n, n), dtype=torch.float64, device="cuda")
+    t = torch.empty_like(u)
+    for _ in range(3):
+        X.line_op_async(1, "der1", 0, u, t)
+    torch.cuda.synchronize()
+    ms = _events_time(stream, lambda: X.line_op_async(1, "der1", 0, u, t), reps)
+    algo = 2.0 * n ** 3 * 8
+    achieved = algo / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(n),
+            "kernel": "line_kernel<double,OP_DER1,32,false> (dery, 256^3 FP64)",
+            "avg_ms": round(ms, 5), "algorithmic_bytes": int(algo)}
+
+
+def _traffic(n):
+    """HBM bytes per launch of the roofline kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        if int(d.get("n", -1)) == n:
+            return d.get("traffic_bytes")
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def cpu_baseline(n, steps=1):
+    """The oracle (numpy, single thread) on a bounded sample of the same
+    workload: `steps` TGV time steps at n^3 after one untimed step."""
+    import numpy as np  # noqa: F401
+    from oracle import x3d_oracle as O
+    p = O.Params(n, n, n, re=1600.0, dt=0.002)
+    S = O.Schemes(p)
+    st = O.tgv_state(p)
+    O.step(S, st)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        O.step(S, st)
+    dt = time.perf_counter() - t0
+    return {"value": round(n ** 3 * steps / dt / 1e6, 4), "unit": "Mpoint-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle (numpy FP64 restatement) TGV {n}^3, {steps} AB3 step(s) after 1 warm-up step, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32", "both"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU slab decomposition is a design, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import incompact3d_amd as X
+
+    n = args.n
+    p = X.make_params(n, n, n, re=1600.0, dt=0.002)
+    X.init(p)
+    stream = torch.cuda.Stream()
+    X.set_stream(stream.cuda_stream)
+    with torch.cuda.stream(stream):
+        r64 = run_solver(X, p, 0, args.warmup, args.steps, stream) if args.only != "fp32" else None
+        r32 = run_solver(X, p, 1, args.warmup, args.steps, stream) if args.only != "fp64" else None
+        roof = roofline_line_kernel(X, n, stream)
+    npts = n ** 3
+    main_r = r64 if r64 is not None else r32
+    ms = main_r["event_ms_per_step"]
+    value = npts / (ms * 1e-3) / 1e6
+    out = {
+        "metric": "Mpoint-steps/s on TGV, FP64 plain vs FP32+learned LES at equal dissipation err",
+        "value": round(value, 2),
+        "unit": "Mpoint-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if r64 is not None else "f32",
+        "data": "synthetic (analytic Taylor-Green initial condition)",
+        "config": {"workload": f"TGV {n}^3 triply periodic, Re=1600, AB3 dt=0.002, 6th-order compact, "
+                               f"spectral Poisson", "grid": [n, n, n], "precision": "fp64"},
+        "roofline": roof,
+    }
+    if r64 is not None:
+        out["fp64"] = {"ms_per_step": round(r64["event_ms_per_step"], 4),
+                       "wall_ms_per_step": round(r64["wall_ms_per_step"], 4),
+                       "tgv_stats": r64["stats"], "div_max_mean": r64["div"]}
+    if r32 is not None:
+        v32 = npts / (r32["event_ms_per_step"] * 1e-3) / 1e6
+        f32 = {"value": round(v32, 2), "ms_per_step": round(r32["event_ms_per_step"], 4),
+               "tgv_stats": r32["stats"], "div_max_mean": r32["div"]}
+        if r64 is not None:
+            f32["speedup_vs_fp64"] = round(v32 / value, 3)
+            e64, e32 = r64["stats"][1], r32["stats"][1]
+            f32["eps_rel_err_vs_fp64"] = abs(e32 - e64) / abs(e64)
+            f32["ke_rel_err_vs_fp64"] = abs(r32["stats"][0] - r64["stats"][0]) / abs(r64["stats"][0])
+        out["fp32_bulk"] = f32
+    out["device"] = X.device_name()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
+        out["cpu_baseline"]["host"] = f"{os.cpu_count()} logical CPUs visible"
+    X.finalize()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -180,6 +180,15 @@ X3D_DECL_DER_FAMILY(float, _f32)
 X3D_DECL_STAG(double, )
 X3D_DECL_STAG(float, _f32)
 
+/* ---- generic asynchronous line operator ----------------------------------
+ * The operator behind every entry above, with the coefficient set the
+ * reference callers pass (ff*p for npaire=1, ...), enqueued on the library
+ * stream without synchronising.  dir 0/1/2 = x/y/z; op 0 der1, 1 der2,
+ * 2 dervp, 3 intervp, 4 derpv, 5 interpv; precision 0 = FP64, 1 = FP32;
+ * (n1,n2,n3) are the extents of the input array. */
+int x3d_line_op_async(int dir, int op, int npaire, int precision, const void *u, void *t, int n1, int n2,
+                      int n3);
+
 /* ---- pressure Poisson -----------------------------------------------------
  * Replaces the procedure pointer `poisson` (poisson.f90:57-65) bound by
  * decomp_2d_poisson_init to poisson_000 (poisson.f90:298) or poisson_010

@@ -22,7 +22,8 @@ from . import derive  # noqa: F401
 
 __all__ = ["Params", "X3DError", "make_params", "init", "finalize", "coef", "poisson",
            "transpose_x_to_y", "transpose_y_to_z", "transpose_z_to_y", "transpose_y_to_x",
-           "reorder_ji", "Solver", "derive", "device_name", "schemes_host"]
+           "reorder_ji", "Solver", "derive", "device_name", "schemes_host", "set_stream",
+           "synchronize", "line_op_async"]
 
 
 def init(p: Params) -> None:
@@ -32,6 +33,29 @@ def init(p: Params) -> None:
 
 def finalize() -> None:
     check(_lib.lib().x3d_finalize(), "x3d_finalize")
+
+
+def set_stream(handle) -> None:
+    """x3d_set_stream: enqueue all library work on this HIP stream handle
+    (e.g. ``torch.cuda.Stream().cuda_stream``); 0/None = default stream."""
+    check(_lib.lib().x3d_set_stream(C.c_void_p(int(handle or 0))), "x3d_set_stream")
+
+
+def synchronize() -> None:
+    check(_lib.lib().x3d_synchronize(), "x3d_synchronize")
+
+
+OPS = {"der1": 0, "der2": 1, "dervp": 2, "intervp": 3, "derpv": 4, "interpv": 5}
+
+
+def line_op_async(dir_: int, op: str, npaire: int, u, t) -> None:
+    """x3d_line_op_async: one line operator (library coefficient sets) on the
+    library stream, no synchronisation.  u, t: device tensors (nz, ny, nx)."""
+    import torch
+    prec = 0 if u.dtype == torch.float64 else 1
+    n3, n2, n1 = u.shape
+    check(_lib.lib().x3d_line_op_async(int(dir_), OPS[op], int(npaire), prec, derive._dptr(u),
+                                       derive._dptr(t), n1, n2, n3), "line_op_async")
 
 
 def device_name() -> str:

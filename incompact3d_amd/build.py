@@ -16,10 +16,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libx3d.so")
-SOURCES = ["capi.cpp", "schemes.cpp", "lines.hip", "fft.hip", "fields.hip", "solver.hip"]
+SOURCES = ["capi.cpp", "schemes.cpp", "lines_f64.hip", "lines_f32.hip", "fft.hip", "fields.hip", "solver.hip"]
 ARCH = os.environ.get("X3D_OFFLOAD_ARCH", "gfx950")
 
-# -packed-fp32-ops: no v_pk_*_f32; -ffp-contract=fast lets
+# -packed-fp32-ops keeps the compiler from emitting packed FP32 arithmetic
+# (v_pk_add/mul/fma_f32); -ffp-contract=fast lets
 # hipcc fuse multiply-adds.
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -82,16 +82,7 @@ int resolve_set(Context &c, int dir, int n, const T *ff, const T *fs, const T *f
   f.alfa = alfa;
   if (periodic) sm_vector(f.ff, f.fs, f.fw, alfa, f.sm, f.sm_den);
   else f.sm.assign(n, 0.0);
-  std::vector<double> h(4 * (size_t)n);
-  std::memcpy(h.data(), f.ff.data(), n * 8);
-  std::memcpy(h.data() + n, f.fs.data(), n * 8);
-  std::memcpy(h.data() + 2 * n, f.fw.data(), n * 8);
-  std::memcpy(h.data() + 3 * n, f.sm.data(), n * 8);
-  std::vector<float> hf(h.begin(), h.end());
-  X3D_HIP(hipMalloc(&f.d64, h.size() * 8));
-  X3D_HIP(hipMalloc(&f.d32, h.size() * 4));
-  X3D_HIP(hipMemcpy(f.d64, h.data(), h.size() * 8, hipMemcpyHostToDevice));
-  X3D_HIP(hipMemcpy(f.d32, hf.data(), hf.size() * 4, hipMemcpyHostToDevice));
+  X3D_TRY(upload_set(f));
   g_foreign.push_back(std::move(f));
   if constexpr (sizeof(T) == 8) *out = g_foreign.back().dev64();
   else *out = g_foreign.back().dev32();
@@ -447,6 +438,32 @@ X3D_DEF_FAMILY(float, _f32)
 
 X3D_DEF_STAG(double, )
 X3D_DEF_STAG(float, _f32)
+
+int x3d_line_op_async(int dir, int op, int npaire, int precision, const void *u, void *t, int n1, int n2,
+                      int n3) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (dir < 0 || dir > 2 || op < 0 || op > 5 || (npaire != 0 && npaire != 1) || !u || !t || n1 <= 0 ||
+      n2 <= 0 || n3 <= 0)
+    return fail(X3D_ERR_ARG, "bad argument");
+  if (precision == 0) {
+    LineCall<double> L;
+    X3D_TRY(make_line_call<double>(*c, dir, op, npaire, &L));
+    L.nx = n1; L.ny = n2; L.nz = n3;
+    L.u = static_cast<const double *>(u);
+    L.t = static_cast<double *>(t);
+    return launch_line<double>(L, c->stream);
+  }
+  if (precision == 1) {
+    LineCall<float> L;
+    X3D_TRY(make_line_call<float>(*c, dir, op, npaire, &L));
+    L.nx = n1; L.ny = n2; L.nz = n3;
+    L.u = static_cast<const float *>(u);
+    L.t = static_cast<float *>(t);
+    return launch_line<float>(L, c->stream);
+  }
+  return fail(X3D_ERR_ARG, "precision must be 0 or 1");
+}
 
 }  // extern "C"
 

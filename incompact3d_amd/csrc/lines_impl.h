@@ -1,3 +1,4 @@
+#pragma once
 // Line-batched compact-scheme solves for gfx950.
 //
 // Every operator of derive.f90 has the same shape along one direction:
@@ -10,18 +11,23 @@
 // GPU mapping (one kernel template for all operators and directions):
 //  * a block owns TL lines x P segments; thread (line, seg) owns L
 //    consecutive points of one line in registers;
-//  * y and z lines are strided in memory, and the TL lines of a block are TL
-//    consecutive i: every load/store of a point row is TL contiguous words
-//    (512 B in FP64), read straight from HBM into registers;
-//  * x lines are contiguous: the block stages TL whole lines through LDS with
-//    fully contiguous loads/stores, and threads read their segment from LDS;
+//  * y and z lines are strided in memory: the TL = 64 lines of a block are 64
+//    consecutive i, so every load/store of a point row is 64 contiguous words
+//    (512 B in FP64) and one wavefront holds one segment of 64 lines -- the
+//    per-point coefficients are then wave-uniform and come in through scalar
+//    loads;
+//  * x lines are contiguous: the block stages TL whole lines (and the
+//    coefficients) through LDS with fully contiguous loads/stores;
 //  * the two first-order recurrences of the Thomas algorithm
-//      y_i = r_i - fs_i y_{i-1},   x_i = (y_i - ff_i x_{i+1}) fw_i
+//      y_i = r_i - fs_i y_{i-1},   x_i = fw_i y_i - (ff_i fw_i) x_{i+1}
 //    are solved per segment with zero carry-in, then the carries are chained
 //    across the P segments of a line through LDS and applied as
 //    y_i += G_i c (G_i = prod(-fs)), x_i += H_i c (H_i = prod(-ff fw)).
 //    |fs|,|ff fw| < 1 for these diagonally dominant systems, so the result
-//    equals the sequential sweep to rounding.
+//    equals the sequential sweep to rounding;
+//  * non-periodic boundary closures (at most 4 rows per end) are evaluated
+//    once per line into LDS before the sweep, so the unrolled interior loop
+//    carries no boundary code.
 #include "x3d_internal.h"
 
 namespace x3d {
@@ -31,18 +37,28 @@ struct LineArgs {
   const T *u, *u2;
   T *t;
   const T *pp;
-  const T *acc, *mul;                 // epilogue: t = ac*acc + rc*mul*result
+  const T *acc, *mul;                 // epilogue: t = ac*acc + rc*(mul*result)
   T ac, rc;
-  const T *ff, *fs, *fw, *sm;
+  const T *pk;                        // packed {fs, fw, ff*fw, r} per point
   T alfa, sm_den;
   Sten<T> s;
-  int periodic, bc1, bcn, npaire;
+  int bc1, bcn, npaire;
   int n_in, n_out;
   int na, nb;                         // line grid extents (a fastest)
   long long sa_in, sb_in, se_in;      // input strides (elements)
   long long sa_out, sb_out, se_out;   // output strides
   int row;                            // LDS row stride (x mode)
 };
+
+// Window offsets an interior row needs: U(i-HL) .. U(i+HR).
+template <int OP>
+struct Halo;
+template <> struct Halo<OP_DER1> { static constexpr int L = 2, R = 2; };
+template <> struct Halo<OP_DER2> { static constexpr int L = 4, R = 4; };
+template <> struct Halo<OP_DERVP> { static constexpr int L = 1, R = 2; };
+template <> struct Halo<OP_INTERVP> { static constexpr int L = 3, R = 4; };
+template <> struct Halo<OP_DERPV> { static constexpr int L = 2, R = 1; };
+template <> struct Halo<OP_INTERPV> { static constexpr int L = 4, R = 3; };
 
 // ---------------------------------------------------------------- stencils
 // Interior rows: v[k] holds U(i) of output row i; offsets are compile-time.
@@ -72,7 +88,7 @@ __device__ __forceinline__ T interior_row(const T (&v)[W], const int k, const St
   }
 }
 
-// Rows of a non-periodic line that use a boundary closure.
+// Rows of a non-periodic line that use a boundary closure (<= 4 per end).
 template <int OP>
 __device__ __forceinline__ bool is_boundary_row(int i, int n_out) {
   if constexpr (OP == OP_DER1) return i < 2 || i >= n_out - 2;
@@ -194,25 +210,29 @@ __device__ T boundary_row(int i, int n_in, int n_out, const Sten<T> &s, int bc1,
 }
 
 // ---------------------------------------------------------------- kernel
-template <typename T, int OP, int L, bool XMODE>
+template <typename T, int OP, int L, bool XMODE, bool PERIODIC>
 __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
+  constexpr int HL = Halo<OP>::L, HR = Halo<OP>::R, W = L + HL + HR;
   const int TL = blockDim.x, P = blockDim.y;
   const int line = threadIdx.x, seg = threadIdx.y;
   const int tid = seg * TL + line, nthr = TL * P;
   const int a = blockIdx.x * TL + line, b = blockIdx.y;
   const bool lv = a < A.na;
   const int n_in = A.n_in, n_out = A.n_out;
-  const int e0 = seg * L;
+  // seg is wave-uniform when TL == 64 (y/z mode): make the compiler see it
+  const int e0 = (XMODE ? seg : __builtin_amdgcn_readfirstlane(seg)) * L;
 
-  T *tile = smem;                                     // XMODE only: TL x row
-  T *carry = smem + (XMODE ? (size_t)TL * A.row : 0); // P*TL
-  T *gend = carry + P * TL;                           // P
-  T *x0s = gend + P;                                  // TL
-  T *xns = x0s + TL;                                  // TL
+  T *tile = smem;                                          // XMODE: TL x row
+  T *cs = tile + (XMODE ? (size_t)TL * A.row : 0);         // XMODE: 4*n_out coefficients
+  T *bnd = cs + (XMODE ? 4 * n_out : 0);                   // !PERIODIC: TL x 8 boundary rows
+  T *carry = bnd + (PERIODIC ? 0 : 8 * TL);                // P*TL
+  T *gend = carry + P * TL;                                // P
+  T *x0s = gend + P;                                       // TL
+  T *xns = x0s + TL;                                       // TL
 
-  // ---- stage x lines through LDS (contiguous tile of TL lines)
+  // ---- x lines (and coefficients) through LDS
   if constexpr (XMODE) {
     const long long base = (long long)blockIdx.x * TL * A.sa_in + (long long)b * A.sb_in;
     const int nv = min(TL, A.na - (int)blockIdx.x * TL);
@@ -224,8 +244,13 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
       if (A.u2) x *= A.u2[base + idx];
       tile[l * A.row + e] = x;
     }
+    for (int idx = tid; idx < 4 * n_out; idx += nthr) cs[idx] = A.pk[idx];
     __syncthreads();
   }
+  auto cf = [&](int e, int j) -> T {
+    if constexpr (XMODE) return cs[4 * e + j];
+    else return A.pk[4 * e + j];
+  };
 
   const long long lbase_in = (long long)a * A.sa_in + (long long)b * A.sb_in;
   auto ldu = [&](int j) -> T {
@@ -239,30 +264,45 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
     }
   };
 
-  // ---- window U(e0-4 .. e0+L+3) into registers
-  T v[L + 8];
+  // ---- boundary closures, once per line, into LDS
+  if constexpr (!PERIODIC) {
+    for (int q = seg; q < 8; q += P) {
+      const int r = q < 4 ? q : n_out - 8 + q;
+      if (lv && r >= 0 && r < n_out && is_boundary_row<OP>(r, n_out))
+        bnd[line * 8 + q] = boundary_row<OP, T>(r, n_in, n_out, A.s, A.bc1, A.bcn, A.npaire, ldu);
+    }
+    __syncthreads();
+  }
+
+  // ---- window U(e0-HL .. e0+L+HR-1) into registers
+  T v[W];
+  const bool active = lv && e0 < n_out;
 #pragma unroll
-  for (int k = 0; k < L + 8; ++k) {
-    int j = e0 - 4 + k;
-    if (A.periodic) {
+  for (int k = 0; k < W; ++k) {
+    int j = e0 - HL + k;
+    if constexpr (PERIODIC) {
       if (j < 0) j += n_in;            // j >= -4 and n_in >= 8
       while (j >= n_in) j -= n_in;     // j < n_in + L + 4
     } else {
       j = j < 0 ? 0 : (j >= n_in ? n_in - 1 : j);
     }
-    v[k] = (lv && e0 < n_out) ? ldu(j) : T(0);
+    v[k] = active ? ldu(j) : T(0);
   }
 
-  // ---- explicit right-hand side, in place (row m only needs v[m..m+8])
+  // ---- explicit right-hand side, in place (row m only needs v[m..m+HL+HR])
 #pragma unroll
   for (int m = 0; m < L; ++m) {
     const int e = e0 + m;
     T r = T(0);
     if (lv && e < n_out) {
-      if (!A.periodic && is_boundary_row<OP>(e, n_out))
-        r = boundary_row<OP, T>(e, n_in, n_out, A.s, A.bc1, A.bcn, A.npaire, ldu);
-      else
-        r = interior_row<OP, T, L + 8>(v, m + 4, A.s);
+      if constexpr (PERIODIC) {
+        r = interior_row<OP, T, W>(v, m + HL, A.s);
+      } else {
+        if (is_boundary_row<OP>(e, n_out))
+          r = bnd[line * 8 + (e < 4 ? e : e - (n_out - 8))];
+        else
+          r = interior_row<OP, T, W>(v, m + HL, A.s);
+      }
     }
     v[m] = r;
   }
@@ -273,29 +313,31 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
   for (int m = 0; m < L; ++m) {
     const int e = e0 + m;
     if (e < n_out) {
-      const T f = A.fs[e];
+      const T f = cf(e, 0);
       prev = v[m] - prev * f;
       v[m] = prev;
       g = -f * g;
     }
   }
-  carry[seg * TL + line] = prev;
-  if (line == 0) gend[seg] = g;
-  __syncthreads();
-  if (seg > 0) {
-    T cin = T(0);
-    for (int s = 0; s < seg; ++s) cin = carry[s * TL + line] + gend[s] * cin;
-    T gg = T(1);
+  if (P > 1) {
+    carry[seg * TL + line] = prev;
+    if (line == 0) gend[seg] = g;
+    __syncthreads();
+    if (seg > 0) {
+      T cin = T(0);
+      for (int s = 0; s < seg; ++s) cin = carry[s * TL + line] + gend[s] * cin;
+      T gg = T(1);
 #pragma unroll
-    for (int m = 0; m < L; ++m) {
-      const int e = e0 + m;
-      if (e < n_out) {
-        gg = -A.fs[e] * gg;
-        v[m] += gg * cin;
+      for (int m = 0; m < L; ++m) {
+        const int e = e0 + m;
+        if (e < n_out) {
+          gg = -cf(e, 0) * gg;
+          v[m] += gg * cin;
+        }
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
 
   // ---- back substitution, local then chained (derive.f90:49-54)
   T nxt = T(0), h = T(1);
@@ -303,31 +345,33 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
   for (int m = L - 1; m >= 0; --m) {
     const int e = e0 + m;
     if (e < n_out) {
-      const T w = A.fw[e], f = A.ff[e];
-      nxt = (v[m] - f * nxt) * w;
+      const T w = cf(e, 1), fw = cf(e, 2);
+      nxt = w * v[m] - fw * nxt;
       v[m] = nxt;
-      h = -(f * w) * h;
+      h = -fw * h;
     }
   }
-  carry[seg * TL + line] = nxt;
-  if (line == 0) gend[seg] = h;
-  __syncthreads();
-  if (seg < P - 1) {
-    T c = T(0);
-    for (int s = P - 1; s > seg; --s) c = carry[s * TL + line] + gend[s] * c;
-    T hh = T(1);
+  if (P > 1) {
+    carry[seg * TL + line] = nxt;
+    if (line == 0) gend[seg] = h;
+    __syncthreads();
+    if (seg < P - 1) {
+      T c = T(0);
+      for (int s = P - 1; s > seg; --s) c = carry[s * TL + line] + gend[s] * c;
+      T hh = T(1);
 #pragma unroll
-    for (int m = L - 1; m >= 0; --m) {
-      const int e = e0 + m;
-      if (e < n_out) {
-        hh = -(A.ff[e] * A.fw[e]) * hh;
-        v[m] += hh * c;
+      for (int m = L - 1; m >= 0; --m) {
+        const int e = e0 + m;
+        if (e < n_out) {
+          hh = -cf(e, 2) * hh;
+          v[m] += hh * c;
+        }
       }
     }
   }
 
   // ---- Sherman-Morrison correction of periodic lines (derive.f90:55-59)
-  if (A.periodic) {
+  if constexpr (PERIODIC) {
     if (seg == 0) x0s[line] = v[0];
 #pragma unroll
     for (int m = 0; m < L; ++m)
@@ -337,7 +381,7 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
 #pragma unroll
     for (int m = 0; m < L; ++m) {
       const int e = e0 + m;
-      if (e < n_out) v[m] = v[m] - sx * A.sm[e];
+      if (e < n_out) v[m] = v[m] - sx * cf(e, 3);
     }
   }
   if (A.pp) {
@@ -348,9 +392,9 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
     }
   }
 
-  // ---- store
+  // ---- store (with the epilogue t = ac*acc + rc*(mul*result))
   if constexpr (XMODE) {
-    // every read of the tile happened before the first barrier above
+    __syncthreads();   // every read of the tile is done
 #pragma unroll
     for (int m = 0; m < L; ++m) {
       const int e = e0 + m;
@@ -388,92 +432,296 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
   }
 }
 
+// ---------------------------------------------------------------- x lines, one wavefront per line
+// Periodic x lines of n = 64*V points: lane k owns points [kV, kV+V) of the
+// line (contiguous, so a wavefront reads/writes the line as one 64*V-word
+// burst), the stencil halo comes from the neighbour lanes through
+// ds_bpermute, and the segment carries of the two recurrences are chained by
+// a Hillis-Steele scan of affine maps across the 64 lanes.  No LDS and no
+// barriers; every wavefront keeps its coefficients in registers and walks a
+// strided sequence of lines.
+template <typename T, int V>
+__device__ __forceinline__ void load_run(const T *__restrict__ p, T (&out)[V]) {
+  if constexpr ((V * sizeof(T)) % 16 == 0) {
+    using V4 = __attribute__((ext_vector_type(16 / sizeof(T)))) T;
+    constexpr int PER = 16 / sizeof(T);
+#pragma unroll
+    for (int q = 0; q < V / PER; ++q) {
+      const V4 x = *reinterpret_cast<const V4 *>(p + q * PER);
+#pragma unroll
+      for (int r = 0; r < PER; ++r) out[q * PER + r] = x[r];
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < V; ++m) out[m] = p[m];
+  }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void store_run(T *__restrict__ p, const T (&in)[V]) {
+  if constexpr ((V * sizeof(T)) % 16 == 0) {
+    using V4 = __attribute__((ext_vector_type(16 / sizeof(T)))) T;
+    constexpr int PER = 16 / sizeof(T);
+#pragma unroll
+    for (int q = 0; q < V / PER; ++q) {
+      V4 x;
+#pragma unroll
+      for (int r = 0; r < PER; ++r) x[r] = in[q * PER + r];
+      *reinterpret_cast<V4 *>(p + q * PER) = x;
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < V; ++m) p[m] = in[m];
+  }
+}
+
+constexpr int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+template <typename T, int OP, int V>
+__global__ void __launch_bounds__(256) xwave_kernel(const LineArgs<T> A, int nlines) {
+  constexpr int HL = Halo<OP>::L, HR = Halo<OP>::R, W = V + HL + HR;
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  const int n = A.n_out;   // == n_in == 64*V
+  const int e0 = lane * V;
+  T fs[V], fw[V], ffw[V], sm[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int e = e0 + m;
+    fs[m] = A.pk[4 * e + 0];
+    fw[m] = A.pk[4 * e + 1];
+    ffw[m] = A.pk[4 * e + 2];
+    sm[m] = A.pk[4 * e + 3];
+  }
+  for (int ln = wave; ln < nlines; ln += nwaves) {
+    const long long base = (long long)ln * n + e0;
+    T own[V];
+    load_run<T, V>(A.u + base, own);
+    if (A.u2) {
+      T o2[V];
+      load_run<T, V>(A.u2 + base, o2);
+#pragma unroll
+      for (int m = 0; m < V; ++m) own[m] *= o2[m];
+    }
+    // window U(e0-HL .. e0+V+HR-1): own points plus neighbour lanes' points
+    T v[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const int d = k - HL;               // offset from e0
+      if (d >= 0 && d < V) {
+        v[k] = own[d];
+      } else {
+        const int lo = floor_div(d, V);    // compile-time after unrolling
+        const int el = d - lo * V;
+        v[k] = __shfl(own[el], (lane + lo) & 63);
+      }
+    }
+    // right-hand side in place
+#pragma unroll
+    for (int m = 0; m < V; ++m) v[m] = interior_row<OP, T, W>(v, m + HL, A.s);
+    // forward elimination: local, then scan of (g, y_end) across lanes
+    T prev = T(0), g = T(1);
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      prev = v[m] - prev * fs[m];
+      v[m] = prev;
+      g = -fs[m] * g;
+    }
+    {
+      T a = g, bb = prev;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const T ap = __shfl_up(a, off), bp = __shfl_up(bb, off);
+        if (lane >= off) {
+          bb = a * bp + bb;
+          a = a * ap;
+        }
+      }
+      T cin = __shfl_up(bb, 1);
+      if (lane == 0) cin = T(0);
+      T gg = T(1);
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        gg = -fs[m] * gg;
+        v[m] += gg * cin;
+      }
+    }
+    // back substitution: local, then reverse scan of (h, x_start)
+    T nxt = T(0), h = T(1);
+#pragma unroll
+    for (int m = V - 1; m >= 0; --m) {
+      nxt = fw[m] * v[m] - ffw[m] * nxt;
+      v[m] = nxt;
+      h = -ffw[m] * h;
+    }
+    {
+      T a = h, bb = nxt;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const T an = __shfl_down(a, off), bn = __shfl_down(bb, off);
+        if (lane + off < 64) {
+          bb = a * bn + bb;
+          a = a * an;
+        }
+      }
+      T c = __shfl_down(bb, 1);
+      if (lane == 63) c = T(0);
+      T hh = T(1);
+#pragma unroll
+      for (int m = V - 1; m >= 0; --m) {
+        hh = -ffw[m] * hh;
+        v[m] += hh * c;
+      }
+    }
+    // Sherman-Morrison correction (derive.f90:55-59)
+    const T x0 = __shfl(v[0], 0), xn = __shfl(v[V - 1], 63);
+    const T sx = (x0 - A.alfa * xn) / A.sm_den;
+#pragma unroll
+    for (int m = 0; m < V; ++m) v[m] = v[m] - sx * sm[m];
+    // epilogue and store
+    T res[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) res[m] = v[m];
+    if (A.mul) {
+      T mm[V];
+      load_run<T, V>(A.mul + base, mm);
+#pragma unroll
+      for (int m = 0; m < V; ++m) res[m] *= mm[m];
+    }
+#pragma unroll
+    for (int m = 0; m < V; ++m) res[m] = A.rc * res[m];
+    if (A.acc) {
+      T aa[V];
+      load_run<T, V>(A.acc + base, aa);
+#pragma unroll
+      for (int m = 0; m < V; ++m) res[m] = A.ac * aa[m] + res[m];
+    }
+    store_run<T, V>(A.t + base, res);
+  }
+}
+
+template <typename T, int OP>
+static int launch_xwave(const LineArgs<T> &a, int nlines, hipStream_t st) {
+  const int V = a.n_out / 64;
+  const int waves = nlines < 16384 ? nlines : 16384;
+  const dim3 grid((waves + 3) / 4), block(256);
+  switch (V) {
+    case 1: hipLaunchKernelGGL((xwave_kernel<T, OP, 1>), grid, block, 0, st, a, nlines); break;
+    case 2: hipLaunchKernelGGL((xwave_kernel<T, OP, 2>), grid, block, 0, st, a, nlines); break;
+    case 4: hipLaunchKernelGGL((xwave_kernel<T, OP, 4>), grid, block, 0, st, a, nlines); break;
+    case 8: hipLaunchKernelGGL((xwave_kernel<T, OP, 8>), grid, block, 0, st, a, nlines); break;
+    default: return fail(X3D_ERR_ARG, "xwave: unsupported line length");
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("xwave launch: ") + hipGetErrorString(e));
+  return X3D_OK;
+}
+
+static inline bool aligned16(const void *p) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
 // ---------------------------------------------------------------- launcher
 template <typename T, int OP, int L>
-static int launch_l(const LineArgs<T> &a, bool xmode, int TL, int P, dim3 grid, size_t shmem,
+static int launch_l(const LineArgs<T> &a, bool xmode, bool periodic, int TL, int P, dim3 grid, size_t shmem,
                     hipStream_t st) {
   dim3 block(TL, P);
-  if (xmode)
-    hipLaunchKernelGGL((line_kernel<T, OP, L, true>), grid, block, shmem, st, a);
-  else
-    hipLaunchKernelGGL((line_kernel<T, OP, L, false>), grid, block, shmem, st, a);
+  if (xmode) {
+    if (periodic) hipLaunchKernelGGL((line_kernel<T, OP, L, true, true>), grid, block, shmem, st, a);
+    else hipLaunchKernelGGL((line_kernel<T, OP, L, true, false>), grid, block, shmem, st, a);
+  } else {
+    if (periodic) hipLaunchKernelGGL((line_kernel<T, OP, L, false, true>), grid, block, shmem, st, a);
+    else hipLaunchKernelGGL((line_kernel<T, OP, L, false, false>), grid, block, shmem, st, a);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("line kernel launch: ") + hipGetErrorString(e));
   return X3D_OK;
 }
 
 template <typename T, int OP>
-static int launch_op(const LineArgs<T> &a, bool xmode, int nmax, dim3 grid_base, hipStream_t st) {
-  int L = nmax <= 128 ? 16 : (nmax <= 512 ? 32 : 64);
-  int P = (nmax + L - 1) / L;
+static int launch_op(const LineArgs<T> &a, bool xmode, bool periodic, int nmax, int nb, hipStream_t st) {
   if (nmax > 1024) return fail(X3D_ERR_UNSUPPORTED, "line length > 1024");
-  int TL = xmode ? 16 : 64;
+  const int L = nmax <= 128 ? 16 : (nmax <= 512 ? 32 : 64);
+  const int P = (nmax + L - 1) / L;
+  int TL = 64;
+  size_t fixed = sizeof(T) * ((size_t)(periodic ? 0 : 8 * TL) + (size_t)P * TL + P + 2 * TL);
+  if (xmode) {
+    // largest power-of-two line count whose tile fits in 64 KB
+    TL = 64;
+    while (TL > 8 && sizeof(T) * (size_t)TL * a.row > 64 * 1024) TL >>= 1;
+    fixed = sizeof(T) * ((size_t)TL * a.row + 4 * (size_t)a.n_out + (periodic ? 0 : 8 * TL) + (size_t)P * TL +
+                         P + 2 * TL);
+  }
   if (TL * P > 1024) return fail(X3D_ERR_UNSUPPORTED, "line kernel block too large");
-  dim3 grid((a.na + TL - 1) / TL, grid_base.y);
-  size_t shmem = sizeof(T) * ((size_t)P * TL + P + 2 * TL);
-  if (xmode) shmem += sizeof(T) * (size_t)TL * a.row;
-  if (shmem > 160 * 1024) return fail(X3D_ERR_UNSUPPORTED, "line kernel LDS > 160 KB");
+  if (fixed > 160 * 1024) return fail(X3D_ERR_UNSUPPORTED, "line kernel LDS > 160 KB");
+  dim3 grid((a.na + TL - 1) / TL, nb);
   switch (L) {
-    case 16: return launch_l<T, OP, 16>(a, xmode, TL, P, grid, shmem, st);
-    case 32: return launch_l<T, OP, 32>(a, xmode, TL, P, grid, shmem, st);
-    default: return launch_l<T, OP, 64>(a, xmode, TL, P, grid, shmem, st);
+    case 16: return launch_l<T, OP, 16>(a, xmode, periodic, TL, P, grid, fixed, st);
+    case 32: return launch_l<T, OP, 32>(a, xmode, periodic, TL, P, grid, fixed, st);
+    default: return launch_l<T, OP, 64>(a, xmode, periodic, TL, P, grid, fixed, st);
   }
 }
 
 template <typename T>
 int launch_line(const LineCall<T> &c, hipStream_t st) {
-  if (!c.u || !c.t || !c.tri.ff || !c.tri.fs || !c.tri.fw) return fail(X3D_ERR_ARG, "null pointer");
-  if (c.tri.periodic && !c.tri.sm) return fail(X3D_ERR_ARG, "periodic set without SM vector");
+  if (!c.u || !c.t || !c.tri.pk) return fail(X3D_ERR_ARG, "null pointer");
   LineArgs<T> a{};
   a.u = c.u; a.u2 = c.u2; a.t = c.t; a.pp = c.pp;
   a.acc = c.acc; a.mul = c.mul; a.ac = c.ac; a.rc = c.rc;
-  a.ff = c.tri.ff; a.fs = c.tri.fs; a.fw = c.tri.fw; a.sm = c.tri.sm;
+  a.pk = c.tri.pk;
   a.alfa = c.tri.alfa; a.sm_den = c.tri.sm_den;
   a.s = c.s;
-  a.periodic = c.tri.periodic; a.bc1 = c.bc1; a.bcn = c.bcn; a.npaire = c.npaire;
+  a.bc1 = c.bc1; a.bcn = c.bcn; a.npaire = c.npaire;
+  const bool periodic = c.tri.periodic != 0;
   const int nx = c.nx, ny = c.ny, nz = c.nz;
   const int n_in = c.dir == 0 ? nx : (c.dir == 1 ? ny : nz);
   const int n_out = c.n_out;
   if (c.tri.n != n_out) return fail(X3D_ERR_ARG, "coefficient length != output line length");
   if (n_in < 8 || n_out < 8) return fail(X3D_ERR_UNSUPPORTED, "lines shorter than 8 points");
-  if (c.tri.periodic && n_in != n_out) return fail(X3D_ERR_ARG, "periodic line with n_in != n_out");
+  if (periodic && n_in != n_out) return fail(X3D_ERR_ARG, "periodic line with n_in != n_out");
   a.n_in = n_in; a.n_out = n_out;
-  dim3 g;
   bool xmode = false;
+  int nb;
   if (c.dir == 0) {
     xmode = true;
-    a.na = ny; a.nb = nz;
+    a.na = ny; a.nb = nz; nb = nz;
     a.se_in = 1; a.sa_in = n_in; a.sb_in = (long long)n_in * ny;
     a.se_out = 1; a.sa_out = n_out; a.sb_out = (long long)n_out * ny;
     int row = (n_in > n_out ? n_in : n_out);
-    if (sizeof(T) == 8) row |= 1;          // odd stride in 8-byte words: no bank conflicts
-    else row = (row % 2 == 0) ? row + 1 : row;
+    if (row % 2 == 0) row += 1;   // odd stride: consecutive lines fall in different LDS banks
     a.row = row;
-    g = dim3(1, nz);
   } else if (c.dir == 1) {
-    a.na = nx; a.nb = nz;
+    a.na = nx; a.nb = nz; nb = nz;
     a.se_in = nx; a.sa_in = 1; a.sb_in = (long long)nx * n_in;
     a.se_out = nx; a.sa_out = 1; a.sb_out = (long long)nx * n_out;
-    g = dim3(1, nz);
   } else {
-    a.na = nx; a.nb = ny;
+    a.na = nx; a.nb = ny; nb = ny;
     a.se_in = (long long)nx * ny; a.sa_in = 1; a.sb_in = nx;
     a.se_out = (long long)nx * ny; a.sa_out = 1; a.sb_out = nx;
-    g = dim3(1, ny);
   }
   const int nmax = n_in > n_out ? n_in : n_out;
+  const int V = n_out / 64;
+  if (xmode && periodic && n_in == n_out && n_out % 64 == 0 && (V == 1 || V == 2 || V == 4 || V == 8) &&
+      !c.pp && aligned16(c.u) && aligned16(c.u2) && aligned16(c.t) && aligned16(c.acc) && aligned16(c.mul)) {
+    const int nlines = ny * nz;
+    switch (c.op) {
+      case OP_DER1: return launch_xwave<T, OP_DER1>(a, nlines, st);
+      case OP_DER2: return launch_xwave<T, OP_DER2>(a, nlines, st);
+      case OP_DERVP: return launch_xwave<T, OP_DERVP>(a, nlines, st);
+      case OP_INTERVP: return launch_xwave<T, OP_INTERVP>(a, nlines, st);
+      case OP_DERPV: return launch_xwave<T, OP_DERPV>(a, nlines, st);
+      case OP_INTERPV: return launch_xwave<T, OP_INTERPV>(a, nlines, st);
+    }
+  }
   switch (c.op) {
-    case OP_DER1: return launch_op<T, OP_DER1>(a, xmode, nmax, g, st);
-    case OP_DER2: return launch_op<T, OP_DER2>(a, xmode, nmax, g, st);
-    case OP_DERVP: return launch_op<T, OP_DERVP>(a, xmode, nmax, g, st);
-    case OP_INTERVP: return launch_op<T, OP_INTERVP>(a, xmode, nmax, g, st);
-    case OP_DERPV: return launch_op<T, OP_DERPV>(a, xmode, nmax, g, st);
-    case OP_INTERPV: return launch_op<T, OP_INTERPV>(a, xmode, nmax, g, st);
+    case OP_DER1: return launch_op<T, OP_DER1>(a, xmode, periodic, nmax, nb, st);
+    case OP_DER2: return launch_op<T, OP_DER2>(a, xmode, periodic, nmax, nb, st);
+    case OP_DERVP: return launch_op<T, OP_DERVP>(a, xmode, periodic, nmax, nb, st);
+    case OP_INTERVP: return launch_op<T, OP_INTERVP>(a, xmode, periodic, nmax, nb, st);
+    case OP_DERPV: return launch_op<T, OP_DERPV>(a, xmode, periodic, nmax, nb, st);
+    case OP_INTERPV: return launch_op<T, OP_INTERPV>(a, xmode, periodic, nmax, nb, st);
   }
   return fail(X3D_ERR_ARG, "unknown line operator");
 }
 
-template int launch_line<double>(const LineCall<double> &, hipStream_t);
-template int launch_line<float>(const LineCall<float> &, hipStream_t);
 
 }  // namespace x3d

@@ -265,18 +265,34 @@ void cast_to(const Sten<double> &a, Sten<T> &b) {
 
 TriDev<double> TriSet::dev64() const {
   TriDev<double> d;
-  const size_t n = ff.size();
-  d.ff = d64; d.fs = d64 + n; d.fw = d64 + 2 * n; d.sm = d64 + 3 * n;
-  d.alfa = alfa; d.sm_den = sm_den; d.n = (int)n; d.periodic = periodic;
+  d.pk = d64;
+  d.alfa = alfa; d.sm_den = sm_den; d.n = (int)ff.size(); d.periodic = periodic;
   return d;
 }
 
 TriDev<float> TriSet::dev32() const {
   TriDev<float> d;
-  const size_t n = ff.size();
-  d.ff = d32; d.fs = d32 + n; d.fw = d32 + 2 * n; d.sm = d32 + 3 * n;
-  d.alfa = (float)alfa; d.sm_den = (float)sm_den; d.n = (int)n; d.periodic = periodic;
+  d.pk = d32;
+  d.alfa = (float)alfa; d.sm_den = (float)sm_den; d.n = (int)ff.size(); d.periodic = periodic;
   return d;
+}
+
+int upload_set(TriSet &s) {
+  const size_t n = s.ff.size();
+  std::vector<double> h(4 * n);
+  for (size_t i = 0; i < n; ++i) {
+    h[4 * i + 0] = s.fs[i];
+    h[4 * i + 1] = s.fw[i];
+    h[4 * i + 2] = s.ff[i] * s.fw[i];
+    h[4 * i + 3] = s.sm[i];
+  }
+  std::vector<float> f(4 * n);
+  for (size_t i = 0; i < 4 * n; ++i) f[i] = (float)h[i];
+  X3D_HIP(hipMalloc(&s.d64, 4 * n * sizeof(double)));
+  X3D_HIP(hipMalloc(&s.d32, 4 * n * sizeof(float)));
+  X3D_HIP(hipMemcpy(s.d64, h.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
+  X3D_HIP(hipMemcpy(s.d32, f.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
+  return X3D_OK;
 }
 
 template <typename T>
@@ -380,20 +396,7 @@ int build_schemes(Context &c) {
 }
 
 int upload_sets(Context &c) {
-  for (TriSet &s : c.sets) {
-    const size_t n = s.ff.size();
-    std::vector<double> h(4 * n);
-    std::memcpy(h.data(), s.ff.data(), n * 8);
-    std::memcpy(h.data() + n, s.fs.data(), n * 8);
-    std::memcpy(h.data() + 2 * n, s.fw.data(), n * 8);
-    std::memcpy(h.data() + 3 * n, s.sm.data(), n * 8);
-    std::vector<float> f(4 * n);
-    for (size_t i = 0; i < 4 * n; ++i) f[i] = (float)h[i];
-    X3D_HIP(hipMalloc(&s.d64, 4 * n * sizeof(double)));
-    X3D_HIP(hipMalloc(&s.d32, 4 * n * sizeof(float)));
-    X3D_HIP(hipMemcpy(s.d64, h.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
-    X3D_HIP(hipMemcpy(s.d32, f.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
-  }
+  for (TriSet &s : c.sets) X3D_TRY(upload_set(s));
   return X3D_OK;
 }
 

@@ -46,10 +46,13 @@ struct Sten {
 
 // One tridiagonal system (Thomas factors of prepare(), schemes.f90:413-439)
 // plus, for periodic lines, the Sherman-Morrison vector r and the
-// denominator 1 + r(1) - alfa*r(n) of derive.f90:55-57.
+// denominator 1 + r(1) - alfa*r(n) of derive.f90:55-57.  On the device the
+// per-point factors are packed as pk[4*i..4*i+3] = {fs, fw, ff*fw, r}: the
+// forward sweep y_i = rhs_i - fs_i y_{i-1}, the backward sweep
+// x_i = fw_i y_i - (ff_i fw_i) x_{i+1}, and the correction x_i -= s r_i.
 template <typename T>
 struct TriDev {
-  const T *ff = nullptr, *fs = nullptr, *fw = nullptr, *sm = nullptr;
+  const T *pk = nullptr;
   T alfa = 0, sm_den = 1;
   int n = 0;
   int periodic = 0;
@@ -61,11 +64,14 @@ struct TriSet {
   std::vector<double> ff, fs, fw, sm;
   double alfa = 0, sm_den = 1;
   int periodic = 0;
-  double *d64 = nullptr;        // 4*n doubles: ff|fs|fw|sm
-  float *d32 = nullptr;         // 4*n floats
+  double *d64 = nullptr;        // packed {fs, fw, ff*fw, r} x n, FP64
+  float *d32 = nullptr;         // the same, FP32
   TriDev<double> dev64() const;
   TriDev<float> dev32() const;
 };
+
+// Upload the packed device copies of one set (allocates d64/d32).
+int upload_set(TriSet &s);
 
 struct DirCoef {
   int n = 0, nm = 0, bc1 = 0, bcn = 0;
