@@ -80,7 +80,7 @@ def roofline_line_kernel(X, n, stream, reps=40):
     achieved = algo / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(n),
-            "kernel": "line_kernel<double,OP_DER1,32,false> (dery, 256^3 FP64)",
+            "kernel": f"x3d::line_kernel<double, 0, 32, 1, true, false> (dery_00, {n}^3 FP64)",
             "avg_ms": round(ms, 5), "algorithmic_bytes": int(algo)}
 
 
@@ -124,9 +124,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32", "both"], default="both")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the roofline kernel (and a copy of known size) -- for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -141,6 +143,18 @@ def main():
     X.init(p)
     stream = torch.cuda.Stream()
     X.set_stream(stream.cuda_stream)
+    if args.roofline_only:
+        with torch.cuda.stream(stream):
+            roof = roofline_line_kernel(X, n, stream, reps=10)
+            # calibration: one pencil copy (transpose_x_to_y) of n^3 FP64 words,
+            # i.e. n^3*8 bytes read and n^3*8 bytes written
+            a = torch.randn((n, n, n), dtype=torch.float64, device="cuda")
+            b = torch.empty_like(a)
+            X.transpose_x_to_y(a, b)
+            torch.cuda.synchronize()
+        X.finalize()
+        print(json.dumps({"roofline": roof, "copy_bytes_each_way": n ** 3 * 8}))
+        return
     with torch.cuda.stream(stream):
         r64 = run_solver(X, p, 0, args.warmup, args.steps, stream) if args.only != "fp32" else None
         r32 = run_solver(X, p, 1, args.warmup, args.steps, stream) if args.only != "fp64" else None

@@ -210,8 +210,11 @@ __device__ T boundary_row(int i, int n_in, int n_out, const Sten<T> &s, int bc1,
 }
 
 // ---------------------------------------------------------------- kernel
-template <typename T, int OP, int L, bool XMODE, bool PERIODIC>
+// AX: 0 x (LDS-staged), 1 y, 2 z (direct); EPI: fused product input and
+// epilogue present (a plain derivative compiles without them).
+template <typename T, int OP, int L, int AX, bool PERIODIC, bool EPI>
 __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
+  constexpr bool XMODE = AX == 0;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
   constexpr int HL = Halo<OP>::L, HR = Halo<OP>::R, W = L + HL + HR;
@@ -241,7 +244,7 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
       const int l = idx / n_in;
       const int e = idx - l * n_in;
       T x = A.u[base + idx];
-      if (A.u2) x *= A.u2[base + idx];
+      if (EPI && A.u2) x *= A.u2[base + idx];
       tile[l * A.row + e] = x;
     }
     for (int idx = tid; idx < 4 * n_out; idx += nthr) cs[idx] = A.pk[idx];
@@ -259,7 +262,7 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
     } else {
       const long long o = lbase_in + (long long)j * A.se_in;
       T x = A.u[o];
-      if (A.u2) x *= A.u2[o];
+      if (EPI && A.u2) x *= A.u2[o];
       return x;
     }
   };
@@ -408,9 +411,9 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
       const int l = idx / n_out;
       const int e = idx - l * n_out;
       T r = tile[l * A.row + e];
-      if (A.mul) r *= A.mul[base + idx];
-      r = A.rc * r;
-      if (A.acc) r = A.ac * A.acc[base + idx] + r;
+      if (EPI && A.mul) r *= A.mul[base + idx];
+      if (EPI) r = A.rc * r;
+      if (EPI && A.acc) r = A.ac * A.acc[base + idx] + r;
       A.t[base + idx] = r;
     }
   } else {
@@ -422,9 +425,9 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
         if (e < n_out) {
           const long long o = lbase_out + (long long)e * A.se_out;
           T r = v[m];
-          if (A.mul) r *= A.mul[o];
-          r = A.rc * r;
-          if (A.acc) r = A.ac * A.acc[o] + r;
+          if (EPI && A.mul) r *= A.mul[o];
+          if (EPI) r = A.rc * r;
+          if (EPI && A.acc) r = A.ac * A.acc[o] + r;
           A.t[o] = r;
         }
       }
@@ -477,7 +480,7 @@ __device__ __forceinline__ void store_run(T *__restrict__ p, const T (&in)[V]) {
 
 constexpr int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
-template <typename T, int OP, int V>
+template <typename T, int OP, int V, bool EPI>
 __global__ void __launch_bounds__(256) xwave_kernel(const LineArgs<T> A, int nlines) {
   constexpr int HL = Halo<OP>::L, HR = Halo<OP>::R, W = V + HL + HR;
   const int lane = threadIdx.x & 63;
@@ -498,7 +501,7 @@ __global__ void __launch_bounds__(256) xwave_kernel(const LineArgs<T> A, int nli
     const long long base = (long long)ln * n + e0;
     T own[V];
     load_run<T, V>(A.u + base, own);
-    if (A.u2) {
+    if (EPI && A.u2) {
       T o2[V];
       load_run<T, V>(A.u2 + base, o2);
 #pragma unroll
@@ -583,15 +586,17 @@ __global__ void __launch_bounds__(256) xwave_kernel(const LineArgs<T> A, int nli
     T res[V];
 #pragma unroll
     for (int m = 0; m < V; ++m) res[m] = v[m];
-    if (A.mul) {
+    if (EPI && A.mul) {
       T mm[V];
       load_run<T, V>(A.mul + base, mm);
 #pragma unroll
       for (int m = 0; m < V; ++m) res[m] *= mm[m];
     }
+    if (EPI) {
 #pragma unroll
-    for (int m = 0; m < V; ++m) res[m] = A.rc * res[m];
-    if (A.acc) {
+      for (int m = 0; m < V; ++m) res[m] = A.rc * res[m];
+    }
+    if (EPI && A.acc) {
       T aa[V];
       load_run<T, V>(A.acc + base, aa);
 #pragma unroll
@@ -601,16 +606,22 @@ __global__ void __launch_bounds__(256) xwave_kernel(const LineArgs<T> A, int nli
   }
 }
 
+template <typename T, int OP, int V>
+static void launch_xv(const LineArgs<T> &a, bool epi, int nlines, dim3 grid, dim3 block, hipStream_t st) {
+  if (epi) hipLaunchKernelGGL((xwave_kernel<T, OP, V, true>), grid, block, 0, st, a, nlines);
+  else hipLaunchKernelGGL((xwave_kernel<T, OP, V, false>), grid, block, 0, st, a, nlines);
+}
+
 template <typename T, int OP>
-static int launch_xwave(const LineArgs<T> &a, int nlines, hipStream_t st) {
+static int launch_xwave(const LineArgs<T> &a, bool epi, int nlines, hipStream_t st) {
   const int V = a.n_out / 64;
   const int waves = nlines < 16384 ? nlines : 16384;
   const dim3 grid((waves + 3) / 4), block(256);
   switch (V) {
-    case 1: hipLaunchKernelGGL((xwave_kernel<T, OP, 1>), grid, block, 0, st, a, nlines); break;
-    case 2: hipLaunchKernelGGL((xwave_kernel<T, OP, 2>), grid, block, 0, st, a, nlines); break;
-    case 4: hipLaunchKernelGGL((xwave_kernel<T, OP, 4>), grid, block, 0, st, a, nlines); break;
-    case 8: hipLaunchKernelGGL((xwave_kernel<T, OP, 8>), grid, block, 0, st, a, nlines); break;
+    case 1: launch_xv<T, OP, 1>(a, epi, nlines, grid, block, st); break;
+    case 2: launch_xv<T, OP, 2>(a, epi, nlines, grid, block, st); break;
+    case 4: launch_xv<T, OP, 4>(a, epi, nlines, grid, block, st); break;
+    case 8: launch_xv<T, OP, 8>(a, epi, nlines, grid, block, st); break;
     default: return fail(X3D_ERR_ARG, "xwave: unsupported line length");
   }
   hipError_t e = hipGetLastError();
@@ -621,24 +632,33 @@ static int launch_xwave(const LineArgs<T> &a, int nlines, hipStream_t st) {
 static inline bool aligned16(const void *p) { return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 // ---------------------------------------------------------------- launcher
-template <typename T, int OP, int L>
-static int launch_l(const LineArgs<T> &a, bool xmode, bool periodic, int TL, int P, dim3 grid, size_t shmem,
-                    hipStream_t st) {
-  dim3 block(TL, P);
-  if (xmode) {
-    if (periodic) hipLaunchKernelGGL((line_kernel<T, OP, L, true, true>), grid, block, shmem, st, a);
-    else hipLaunchKernelGGL((line_kernel<T, OP, L, true, false>), grid, block, shmem, st, a);
+template <typename T, int OP, int L, int AX>
+static void launch_ax(const LineArgs<T> &a, bool periodic, bool epi, dim3 grid, dim3 block, size_t shmem,
+                      hipStream_t st) {
+  if (periodic) {
+    if (epi) hipLaunchKernelGGL((line_kernel<T, OP, L, AX, true, true>), grid, block, shmem, st, a);
+    else hipLaunchKernelGGL((line_kernel<T, OP, L, AX, true, false>), grid, block, shmem, st, a);
   } else {
-    if (periodic) hipLaunchKernelGGL((line_kernel<T, OP, L, false, true>), grid, block, shmem, st, a);
-    else hipLaunchKernelGGL((line_kernel<T, OP, L, false, false>), grid, block, shmem, st, a);
+    if (epi) hipLaunchKernelGGL((line_kernel<T, OP, L, AX, false, true>), grid, block, shmem, st, a);
+    else hipLaunchKernelGGL((line_kernel<T, OP, L, AX, false, false>), grid, block, shmem, st, a);
   }
+}
+
+template <typename T, int OP, int L>
+static int launch_l(const LineArgs<T> &a, int ax, bool periodic, bool epi, int TL, int P, dim3 grid,
+                    size_t shmem, hipStream_t st) {
+  dim3 block(TL, P);
+  if (ax == 0) launch_ax<T, OP, L, 0>(a, periodic, epi, grid, block, shmem, st);
+  else if (ax == 1) launch_ax<T, OP, L, 1>(a, periodic, epi, grid, block, shmem, st);
+  else launch_ax<T, OP, L, 2>(a, periodic, epi, grid, block, shmem, st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("line kernel launch: ") + hipGetErrorString(e));
   return X3D_OK;
 }
 
 template <typename T, int OP>
-static int launch_op(const LineArgs<T> &a, bool xmode, bool periodic, int nmax, int nb, hipStream_t st) {
+static int launch_op(const LineArgs<T> &a, int ax, bool periodic, bool epi, int nmax, int nb, hipStream_t st) {
+  const bool xmode = ax == 0;
   if (nmax > 1024) return fail(X3D_ERR_UNSUPPORTED, "line length > 1024");
   const int L = nmax <= 128 ? 16 : (nmax <= 512 ? 32 : 64);
   const int P = (nmax + L - 1) / L;
@@ -655,9 +675,9 @@ static int launch_op(const LineArgs<T> &a, bool xmode, bool periodic, int nmax, 
   if (fixed > 160 * 1024) return fail(X3D_ERR_UNSUPPORTED, "line kernel LDS > 160 KB");
   dim3 grid((a.na + TL - 1) / TL, nb);
   switch (L) {
-    case 16: return launch_l<T, OP, 16>(a, xmode, periodic, TL, P, grid, fixed, st);
-    case 32: return launch_l<T, OP, 32>(a, xmode, periodic, TL, P, grid, fixed, st);
-    default: return launch_l<T, OP, 64>(a, xmode, periodic, TL, P, grid, fixed, st);
+    case 16: return launch_l<T, OP, 16>(a, ax, periodic, epi, TL, P, grid, fixed, st);
+    case 32: return launch_l<T, OP, 32>(a, ax, periodic, epi, TL, P, grid, fixed, st);
+    default: return launch_l<T, OP, 64>(a, ax, periodic, epi, TL, P, grid, fixed, st);
   }
 }
 
@@ -699,26 +719,28 @@ int launch_line(const LineCall<T> &c, hipStream_t st) {
     a.se_out = (long long)nx * ny; a.sa_out = 1; a.sb_out = nx;
   }
   const int nmax = n_in > n_out ? n_in : n_out;
+  const bool epi = c.u2 || c.acc || c.mul || c.rc != T(1);
   const int V = n_out / 64;
   if (xmode && periodic && n_in == n_out && n_out % 64 == 0 && (V == 1 || V == 2 || V == 4 || V == 8) &&
       !c.pp && aligned16(c.u) && aligned16(c.u2) && aligned16(c.t) && aligned16(c.acc) && aligned16(c.mul)) {
     const int nlines = ny * nz;
     switch (c.op) {
-      case OP_DER1: return launch_xwave<T, OP_DER1>(a, nlines, st);
-      case OP_DER2: return launch_xwave<T, OP_DER2>(a, nlines, st);
-      case OP_DERVP: return launch_xwave<T, OP_DERVP>(a, nlines, st);
-      case OP_INTERVP: return launch_xwave<T, OP_INTERVP>(a, nlines, st);
-      case OP_DERPV: return launch_xwave<T, OP_DERPV>(a, nlines, st);
-      case OP_INTERPV: return launch_xwave<T, OP_INTERPV>(a, nlines, st);
+      case OP_DER1: return launch_xwave<T, OP_DER1>(a, epi, nlines, st);
+      case OP_DER2: return launch_xwave<T, OP_DER2>(a, epi, nlines, st);
+      case OP_DERVP: return launch_xwave<T, OP_DERVP>(a, epi, nlines, st);
+      case OP_INTERVP: return launch_xwave<T, OP_INTERVP>(a, epi, nlines, st);
+      case OP_DERPV: return launch_xwave<T, OP_DERPV>(a, epi, nlines, st);
+      case OP_INTERPV: return launch_xwave<T, OP_INTERPV>(a, epi, nlines, st);
     }
   }
+  const int ax = c.dir;
   switch (c.op) {
-    case OP_DER1: return launch_op<T, OP_DER1>(a, xmode, periodic, nmax, nb, st);
-    case OP_DER2: return launch_op<T, OP_DER2>(a, xmode, periodic, nmax, nb, st);
-    case OP_DERVP: return launch_op<T, OP_DERVP>(a, xmode, periodic, nmax, nb, st);
-    case OP_INTERVP: return launch_op<T, OP_INTERVP>(a, xmode, periodic, nmax, nb, st);
-    case OP_DERPV: return launch_op<T, OP_DERPV>(a, xmode, periodic, nmax, nb, st);
-    case OP_INTERPV: return launch_op<T, OP_INTERPV>(a, xmode, periodic, nmax, nb, st);
+    case OP_DER1: return launch_op<T, OP_DER1>(a, ax, periodic, epi, nmax, nb, st);
+    case OP_DER2: return launch_op<T, OP_DER2>(a, ax, periodic, epi, nmax, nb, st);
+    case OP_DERVP: return launch_op<T, OP_DERVP>(a, ax, periodic, epi, nmax, nb, st);
+    case OP_INTERVP: return launch_op<T, OP_INTERVP>(a, ax, periodic, epi, nmax, nb, st);
+    case OP_DERPV: return launch_op<T, OP_DERPV>(a, ax, periodic, epi, nmax, nb, st);
+    case OP_INTERPV: return launch_op<T, OP_INTERPV>(a, ax, periodic, epi, nmax, nb, st);
   }
   return fail(X3D_ERR_ARG, "unknown line operator");
 }

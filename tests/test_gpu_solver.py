@@ -56,6 +56,24 @@ def test_tgv_steps_fp64(tgv32):
     s.close()
 
 
+def test_tgv32_200_steps_match_oracle(tgv32):
+    """200 FP64 steps on the GPU against 200 oracle steps (same 32^3 case)."""
+    p, S, X = tgv32
+    s = X.Solver(X.make_params(nx=p.nx, ny=p.ny, nz=p.nz, re=p.re, dt=p.dt), 0)
+    s.init_tgv()
+    s.step(200)
+    st = O.tgv_state(p)
+    for _ in range(200):
+        O.step(S, st)
+    got = s.tgv_stats()
+    ref = O.tgv_diagnostics(S, st.ux, st.uy, st.uz)
+    for g, r in zip(got, ref):
+        assert abs(g - r) <= 1e-10 * abs(r), (got, ref)
+    ux = s.velocity()[0].cpu().numpy()
+    assert rel_err(ux, st.ux) <= 1e-10
+    s.close()
+
+
 def test_tgv_steps_fp32_bulk(tgv32):
     p, S, X = tgv32
     s = X.Solver(X.make_params(nx=p.nx, ny=p.ny, nz=p.nz, re=p.re, dt=p.dt), 1)
