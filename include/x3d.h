@@ -224,7 +224,11 @@ int x3d_host_poisson(double *rhs, const int *n1, const int *n2, const int *n3);
  * time_integrators.f90:85-100) and all work arrays in HBM.
  * precision: 0 = FP64 everywhere; 1 = FP32 bulk (FP32 derivatives, FFTs,
  * convective products) with FP64 velocity accumulation, FP64 Poisson divide
- * and FP64 diagnostics reductions. */
+ * and FP64 diagnostics reductions.
+ * For nx in {64,128,256,512} and ny, nz <= 512 the momentum right-hand side
+ * and AB3 run as nine fused sweeps; other grids (or the environment variable
+ * X3D_UNFUSED_MOMENTUM set at create time) use one launch per derivative.
+ * Both paths compute the same quantities. */
 typedef struct x3d_solver x3d_solver;
 int x3d_solver_create(int precision, x3d_solver **out);
 int x3d_solver_destroy(x3d_solver *s);

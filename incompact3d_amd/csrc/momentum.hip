@@ -1,0 +1,541 @@
+// Fused momentum right-hand side of transeq.f90:73-470 for a periodic box.
+//
+// One launch per (sweep direction d, velocity component c) computes the
+// three compact solves the reference does for that pair
+//     D1_d(u_d u_c)   (transeq.f90:125/198/259)
+//     D1_d(u_c)       (transeq.f90:128/201/262)
+//     D2_d(u_c)       (transeq.f90:301/336/442)
+// and accumulates  -1/2 D1_d(u_d u_c) - 1/2 u_d D1_d(u_c) + nu D2_d(u_c)
+// into the running right-hand side, so each (d, c) pair moves four fields
+// through HBM (u_d, u_c, the accumulator in, the accumulator out) instead of
+// ten.  The three solves run one after the other on the same registers; their
+// input windows are re-read from L2.  The x sweep, the last one in the
+// reference's order, also applies AB3 (time_integrators.f90:85-100): the new
+// velocity goes to a second buffer, because the other components still read
+// the old one.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "lines_impl.h"
+
+namespace x3d {
+
+template <typename T>
+struct MomArgs {
+  const T *ud, *uc, *acc;
+  T *out;
+  const T *pk1, *pk2;           // packed first/second-derivative sets
+  T alfa1, den1, alfa2, den2;
+  Sten<T> s;
+  T xnu;
+  int n, na, nb;
+  long long sa, sb, se;         // line strides (y/z sweeps)
+  int kdepth;                   // carry-chain depth in segments (y/z sweeps)
+  // AB3 epilogue (x sweep only): u_new = ca*f1 + cb*f2 + cc*f3 + u_old
+  const T *f2, *f3;
+  const double *u_old;
+  double *u_new;
+  float *shadow;                // FP32 copy of u_new (FP32-bulk path)
+  double ca, cb, cc;
+  int nterms;                   // 0: no AB3
+};
+
+// ---------------------------------------------------------------- y / z sweeps
+// Block: TL consecutive i (one row of TL words per line point) x P segments
+// of L points; a thread owns one segment of one line.  The two recurrences
+// of each solve run locally per segment with a zero carry-in; the carries
+// are then chained through LDS.  The chain stops K segments back: the factor
+// a carry picks up across a segment is the product of that segment's
+// elimination factors, and K is chosen on the host so that the product over
+// any K consecutive segments is below the precision's resolution (the terms
+// dropped are smaller than the rounding of the full chain).
+template <typename T, int L, int NR>
+__device__ __forceinline__ void seg_solve(T (&v)[NR][L], const int e0, const T *__restrict__ cs, const T alfa,
+                                          const T den, T *cf, T *cb, T *gf, T *gb, T *x0s, T *xns, const int line,
+                                          const int seg, const int TL, const int P, const int K) {
+  // NR right-hand sides share the matrix: each coefficient is read once.
+  const int PT = P * TL;
+  // forward elimination, zero carry-in
+  T prev[NR], g = T(1);
+#pragma unroll
+  for (int q = 0; q < NR; ++q) prev[q] = T(0);
+#pragma unroll
+  for (int m = 0; m < L; ++m) {
+    const T f = cs[4 * (e0 + m)];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      prev[q] = v[q][m] - prev[q] * f;
+      v[q][m] = prev[q];
+    }
+    g = -f * g;
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) cf[q * PT + seg * TL + line] = prev[q];
+  if (line == 0) gf[seg] = g;
+  __syncthreads();
+  if (seg > 0) {
+    T cin[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) cin[q] = T(0);
+    for (int s = seg - K > 0 ? seg - K : 0; s < seg; ++s) {
+      const T gs = gf[s];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) cin[q] = cf[q * PT + s * TL + line] + gs * cin[q];
+    }
+    T gg = T(1);
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      gg = -cs[4 * (e0 + m)] * gg;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) v[q][m] += gg * cin[q];
+    }
+  }
+  // back substitution, zero carry-in
+  T nxt[NR], h = T(1);
+#pragma unroll
+  for (int q = 0; q < NR; ++q) nxt[q] = T(0);
+#pragma unroll
+  for (int m = L - 1; m >= 0; --m) {
+    const T w = cs[4 * (e0 + m) + 1], ffw = cs[4 * (e0 + m) + 2];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      nxt[q] = w * v[q][m] - ffw * nxt[q];
+      v[q][m] = nxt[q];
+    }
+    h = -ffw * h;
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) cb[q * PT + seg * TL + line] = nxt[q];
+  if (line == 0) gb[seg] = h;
+  __syncthreads();
+  if (seg < P - 1) {
+    T c[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) c[q] = T(0);
+    for (int s = seg + K < P - 1 ? seg + K : P - 1; s > seg; --s) {
+      const T gs = gb[s];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) c[q] = cb[q * PT + s * TL + line] + gs * c[q];
+    }
+    T hh = T(1);
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) {
+      hh = -cs[4 * (e0 + m) + 2] * hh;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) v[q][m] += hh * c[q];
+    }
+  }
+  // periodic closure (Sherman-Morrison with the precomputed vector sm)
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    if (seg == 0) x0s[q * TL + line] = v[q][0];
+    if (seg == P - 1) xns[q * TL + line] = v[q][L - 1];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const T sx = (x0s[q * TL + line] - alfa * xns[q * TL + line]) / den;
+#pragma unroll
+    for (int m = 0; m < L; ++m) v[q][m] = v[q][m] - sx * cs[4 * (e0 + m) + 3];
+  }
+  // No barrier needed before the next solve: it writes cf/gf before its
+  // first barrier (last read here before the second), cb/gb after its first
+  // barrier (last read here before the third) and x0s/xns after its second.
+}
+
+template <typename T>
+__device__ __forceinline__ T &at(T *p, unsigned byte_off) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ const T &at(const T *p, unsigned byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + byte_off);
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(1024) mom_yz_kernel(const MomArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  const int TL = blockDim.x, P = blockDim.y;
+  const int line = threadIdx.x, seg = threadIdx.y;
+  const int tid = seg * TL + line, nthr = TL * P;
+  const int n = A.n;
+  // LDS: two coefficient tables, carries of two right-hand sides
+  T *cs1 = smem, *cs2 = cs1 + 4 * n, *cf = cs2 + 4 * n, *cb = cf + 2 * P * TL;
+  T *gf = cb + 2 * P * TL, *gb = gf + P, *x0s = gb + P, *xns = x0s + 2 * TL;
+  for (int i = tid; i < 4 * n; i += nthr) {
+    cs1[i] = A.pk1[i];
+    cs2[i] = A.pk2[i];
+  }
+  __syncthreads();
+  const int a = blockIdx.x * TL + line, b = blockIdx.y;
+  const bool active = a < A.na;
+  const int e0 = seg * L;        // n = P*L: every segment is full
+  // byte offsets in 32 bits (the host checks the field size), so each load is
+  // a uniform base plus one VGPR offset
+  const unsigned base = ((unsigned)a * (unsigned)A.sa + (unsigned)b * (unsigned)A.sb) * (unsigned)sizeof(T);
+  const unsigned step = (unsigned)A.se * (unsigned)sizeof(T);
+  // windows reach at most 4 points past either end of the line (n >= 8)
+  auto wrap = [&](int j) { return (unsigned)(j < 0 ? j + n : (j >= n ? j - n : j)); };
+  // right-hand sides: w[0] of D1(u_d u_c), w[1] of D1(u_c), w2 of D2(u_c);
+  // u_c is read over [e0-4, e0+L+4), u_d over [e0-2, e0+L+2), each once
+  T w[2][L], w2[1][L], ud[L];
+  {
+    T wc[L + 8];
+#pragma unroll
+    for (int k = 0; k < L + 8; ++k) wc[k] = active ? at(A.uc, base + wrap(e0 - 4 + k) * step) : T(0);
+    {
+      T p[L + 4];
+#pragma unroll
+      for (int k = 0; k < L + 4; ++k) {
+        const T d = active ? at(A.ud, base + wrap(e0 - 2 + k) * step) : T(0);
+        p[k] = d * wc[k + 2];
+        if (k >= 2 && k < L + 2) ud[k - 2] = d;
+      }
+#pragma unroll
+      for (int m = 0; m < L; ++m) w[0][m] = interior_row<OP_DER1, T, L + 4>(p, m + 2, A.s);
+    }
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      w[1][m] = interior_row<OP_DER1, T, L + 8>(wc, m + 4, A.s);
+      w2[0][m] = interior_row<OP_DER2, T, L + 8>(wc, m + 4, A.s);
+    }
+  }
+  seg_solve<T, L, 2>(w, e0, cs1, A.alfa1, A.den1, cf, cb, gf, gb, x0s, xns, line, seg, TL, P, A.kdepth);
+  seg_solve<T, L, 1>(w2, e0, cs2, A.alfa2, A.den2, cf, cb, gf, gb, x0s, xns, line, seg, TL, P, A.kdepth);
+  if (active) {
+    // acc may be out (in-place accumulation): all of this thread's acc
+    // points are read before any is written
+    T ac[L];
+#pragma unroll
+    for (int m = 0; m < L; ++m) ac[m] = A.acc ? at(A.acc, base + (unsigned)(e0 + m) * step) : T(0);
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      // -1/2 D1(u_d u_c) - 1/2 u_d D1(u_c) + nu D2(u_c)
+      const T r = T(-0.5) * w[0][m] + T(-0.5) * (ud[m] * w[1][m]) + A.xnu * w2[0][m];
+      at(A.out, base + (unsigned)(e0 + m) * step) = ac[m] + r;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- x sweep (one wavefront per line)
+template <typename T, int V, int HL, int HR>
+__device__ __forceinline__ void halo_window(const T (&own)[V], T (&v)[V + HL + HR], const int lane) {
+#pragma unroll
+  for (int k = 0; k < V + HL + HR; ++k) {
+    const int d = k - HL;
+    if (d >= 0 && d < V) {
+      v[k] = own[d];
+    } else {
+      const int lo = floor_div(d, V);
+      const int el = d - lo * V;
+      v[k] = __shfl(own[el], (lane + lo) & 63);
+    }
+  }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void wave_solve(T (&v)[V], const T (&fs)[V], const T (&fw)[V], const T (&ffw)[V],
+                                           const T (&sm)[V], const T alfa, const T den, const int lane) {
+  T prev = T(0), g = T(1);
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    prev = v[m] - prev * fs[m];
+    v[m] = prev;
+    g = -fs[m] * g;
+  }
+  {
+    T a = g, bb = prev;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const T ap = __shfl_up(a, off), bp = __shfl_up(bb, off);
+      if (lane >= off) {
+        bb = a * bp + bb;
+        a = a * ap;
+      }
+    }
+    T cin = __shfl_up(bb, 1);
+    if (lane == 0) cin = T(0);
+    T gg = T(1);
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      gg = -fs[m] * gg;
+      v[m] += gg * cin;
+    }
+  }
+  T nxt = T(0), h = T(1);
+#pragma unroll
+  for (int m = V - 1; m >= 0; --m) {
+    nxt = fw[m] * v[m] - ffw[m] * nxt;
+    v[m] = nxt;
+    h = -ffw[m] * h;
+  }
+  {
+    T a = h, bb = nxt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const T an = __shfl_down(a, off), bn = __shfl_down(bb, off);
+      if (lane + off < 64) {
+        bb = a * bn + bb;
+        a = a * an;
+      }
+    }
+    T c = __shfl_down(bb, 1);
+    if (lane == 63) c = T(0);
+    T hh = T(1);
+#pragma unroll
+    for (int m = V - 1; m >= 0; --m) {
+      hh = -ffw[m] * hh;
+      v[m] += hh * c;
+    }
+  }
+  const T x0 = __shfl(v[0], 0), xn = __shfl(v[V - 1], 63);
+  const T sx = (x0 - alfa * xn) / den;
+#pragma unroll
+  for (int m = 0; m < V; ++m) v[m] = v[m] - sx * sm[m];
+}
+
+template <typename T, int V>
+__global__ void __launch_bounds__(256) mom_x_kernel(const MomArgs<T> A, int nlines) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  const int n = A.n;
+  const int e0 = lane * V;
+  T fs1[V], fw1[V], ffw1[V], sm1[V], fs2[V], fw2[V], ffw2[V], sm2[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    const int e = e0 + m;
+    fs1[m] = A.pk1[4 * e]; fw1[m] = A.pk1[4 * e + 1]; ffw1[m] = A.pk1[4 * e + 2]; sm1[m] = A.pk1[4 * e + 3];
+    fs2[m] = A.pk2[4 * e]; fw2[m] = A.pk2[4 * e + 1]; ffw2[m] = A.pk2[4 * e + 2]; sm2[m] = A.pk2[4 * e + 3];
+  }
+  for (int ln = wave; ln < nlines; ln += nwaves) {
+    const long long base = (long long)ln * n + e0;
+    T ud[V], uc[V];
+    load_run<T, V>(A.ud + base, ud);
+    load_run<T, V>(A.uc + base, uc);
+    T r[V];
+    {
+      T p[V];
+#pragma unroll
+      for (int m = 0; m < V; ++m) p[m] = ud[m] * uc[m];
+      T v[V + 4];
+      halo_window<T, V, 2, 2>(p, v, lane);
+      T w[V];
+#pragma unroll
+      for (int m = 0; m < V; ++m) w[m] = A.s.afi * (v[m + 3] - v[m + 1]) + A.s.bfi * (v[m + 4] - v[m]);
+      wave_solve<T, V>(w, fs1, fw1, ffw1, sm1, A.alfa1, A.den1, lane);
+#pragma unroll
+      for (int m = 0; m < V; ++m) r[m] = T(-0.5) * w[m];
+    }
+    {
+      T v[V + 4];
+      halo_window<T, V, 2, 2>(uc, v, lane);
+      T w[V];
+#pragma unroll
+      for (int m = 0; m < V; ++m) w[m] = A.s.afi * (v[m + 3] - v[m + 1]) + A.s.bfi * (v[m + 4] - v[m]);
+      wave_solve<T, V>(w, fs1, fw1, ffw1, sm1, A.alfa1, A.den1, lane);
+#pragma unroll
+      for (int m = 0; m < V; ++m) r[m] += T(-0.5) * (ud[m] * w[m]);
+    }
+    {
+      T v[V + 8];
+      halo_window<T, V, 4, 4>(uc, v, lane);
+      T w[V];
+#pragma unroll
+      for (int m = 0; m < V; ++m) w[m] = interior_row<OP_DER2, T, V + 8>(v, m + 4, A.s);
+      wave_solve<T, V>(w, fs2, fw2, ffw2, sm2, A.alfa2, A.den2, lane);
+#pragma unroll
+      for (int m = 0; m < V; ++m) r[m] += A.xnu * w[m];
+    }
+    // every epilogue input is read before the first store (the compiler
+    // cannot move loads above stores to possibly aliasing arrays)
+    T acc[V], f2[V], f3[V];
+    double uo[V];
+    if (A.acc) load_run<T, V>(A.acc + base, acc);
+    if (A.nterms) load_run<double, V>(A.u_old + base, uo);
+    if (A.nterms >= 2) load_run<T, V>(A.f2 + base, f2);
+    if (A.nterms >= 3) load_run<T, V>(A.f3 + base, f3);
+    if (A.acc) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) r[m] = acc[m] + r[m];
+    }
+    store_run<T, V>(A.out + base, r);
+    if (A.nterms) {
+      // AB3 with FP64 accumulation: u_new = a f1 + b f2 + c f3 + u_old
+      double un[V];
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        if (A.nterms == 1) un[m] = A.ca * (double)r[m] + uo[m];
+        else if (A.nterms == 2) un[m] = A.ca * (double)r[m] + A.cb * (double)f2[m] + uo[m];
+        else un[m] = A.ca * (double)r[m] + A.cb * (double)f2[m] + A.cc * (double)f3[m] + uo[m];
+      }
+      store_run<double, V>(A.u_new + base, un);
+      if (A.shadow) {
+        float sh[V];
+#pragma unroll
+        for (int m = 0; m < V; ++m) sh[m] = (float)un[m];
+        store_run<float, V>(A.shadow + base, sh);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- host side
+namespace {
+bool same_set(const TriSet &a, const TriSet &b) {
+  return &a == &b || (a.ff == b.ff && a.fs == b.fs && a.fw == b.fw && a.sm == b.sm && a.alfa == b.alfa &&
+                      a.sm_den == b.sm_den);
+}
+// Launch shape of the y/z sweep for a line of n points: segments of L = 8
+// (FP64) or 16 (FP32) points, so that the three solves keep their windows,
+// right-hand sides and results within 128 VGPRs; TL lines per block with
+// TL*P <= 1024 threads and the LDS (two coefficient tables, the carries of
+// two right-hand sides) within 64 KiB.
+template <typename T>
+struct YzShape {
+  static constexpr int L = sizeof(T) == 8 ? 8 : 16;
+  int P = 0, TL = 0;
+  size_t shmem = 0;
+  static size_t lds(int n, int P, int TL) {
+    return sizeof(T) * (8 * (size_t)n + 4 * (size_t)P * TL + 2 * (size_t)P + 4 * (size_t)TL);
+  }
+  bool plan(int n) {
+    if (n < 8 || n % L != 0) return false;
+    P = n / L;
+    TL = 64;
+    while (TL >= 8 && (TL * P > 1024 || lds(n, P, TL) > 65536)) TL >>= 1;
+    if (TL < 8) return false;
+    shmem = lds(n, P, TL);
+    return true;
+  }
+};
+
+// Smallest K such that the elimination factors of any K consecutive segments
+// of L points multiply to less than tol*(1-rho) in magnitude, rho being the
+// largest single-segment product: the carries of segments further back then
+// change no result by more than tol times the largest carry.  Forward factors
+// are fs, backward factors ff*fw (the recurrences of prepare, schemes.f90:
+// 413-439).  Returns P (the full chain) when no shorter depth qualifies.
+int chain_depth(const TriSet &s, int L, double tol) {
+  const int n = (int)s.fs.size();
+  const int P = (n + L - 1) / L;
+  std::vector<double> gfw(P, 1.0), gbw(P, 1.0);
+  for (int e = 0; e < n; ++e) {
+    gfw[e / L] *= std::fabs(s.fs[e]);
+    gbw[e / L] *= std::fabs(s.ff[e] * s.fw[e]);
+  }
+  double rho = 0;
+  for (int p = 0; p < P; ++p) rho = std::max(rho, std::max(gfw[p], gbw[p]));
+  if (rho >= 1.0) return P;
+  for (int K = 1; K < P; ++K) {
+    double worst = 0;
+    for (int p0 = 0; p0 + K <= P; ++p0) {
+      double pf = 1, pb = 1;
+      for (int q = p0; q < p0 + K; ++q) {
+        pf *= gfw[q];
+        pb *= gbw[q];
+      }
+      worst = std::max(worst, std::max(pf, pb));
+    }
+    if (worst < tol * (1.0 - rho)) return K;
+  }
+  return P;
+}
+}  // namespace
+
+// The fused sweeps cover periodic lines with nx = 64, 128, 256 or 512 (one
+// wavefront per x line, 1..8 points per lane) and ny, nz that YzShape can
+// plan (a multiple of 16 whose block fits; 16..512 for both precisions).  The product derivative
+// uses the set of parity d==c (transeq.f90:125) and the velocity derivative
+// the opposite one; one set serves both, so they must hold the same
+// coefficients, which is the case for periodic lines (schemes.f90:583-596
+// builds ffp from ff and prepares both from the same diagonals when
+// ncl1 = ncln = 0).
+bool momentum_fusable(const Context &c) {
+  for (int d = 0; d < 3; ++d) {
+    const DirCoef &D = c.dir[d];
+    if (D.bc1 != 0 || D.bcn != 0) return false;
+    if (!same_set(c.sets[D.set_d1_even], c.sets[D.set_d1_odd])) return false;
+    if (d == 0) {
+      const int V = D.n / 64;
+      if (D.n % 64 != 0 || !(V == 1 || V == 2 || V == 4 || V == 8)) return false;
+    } else if (!YzShape<double>().plan(D.n) || !YzShape<float>().plan(D.n)) {
+      return false;
+    }
+  }
+  return true;
+}
+
+template <typename T>
+int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], const T *acc, T *out,
+                    const T *f2, const T *f3, const double *u_old, double *u_new, float *shadow, double ca,
+                    double cb, double cc, int nterms, int nx, int ny, int nz, hipStream_t st) {
+  const DirCoef &D = c.dir[dir];
+  if (D.bc1 != 0) return fail(X3D_ERR_UNSUPPORTED, "fused momentum sweep: periodic lines only");
+  if (!same_set(c.sets[D.set_d1_even], c.sets[D.set_d1_odd]))
+    return fail(X3D_ERR_UNSUPPORTED, "fused momentum sweep: product and velocity sets differ");
+  const int pv = dir == comp ? 0 : 1;
+  const TriSet &s1 = c.sets[pv == 1 ? D.set_d1_even : D.set_d1_odd];
+  const TriSet &s2 = c.sets[pv == 1 ? D.set_d2_even : D.set_d2_odd];
+  MomArgs<T> A{};
+  A.ud = u[dir];
+  A.uc = u[comp];
+  A.acc = acc;
+  A.out = out;
+  TriDev<T> t1, t2;
+  if constexpr (sizeof(T) == 8) { t1 = s1.dev64(); t2 = s2.dev64(); }
+  else { t1 = s1.dev32(); t2 = s2.dev32(); }
+  A.pk1 = t1.pk; A.pk2 = t2.pk;
+  A.alfa1 = t1.alfa; A.den1 = t1.sm_den; A.alfa2 = t2.alfa; A.den2 = t2.sm_den;
+  A.s = cast_sten<T>(D.s);
+  A.xnu = (T)c.xnu;
+  A.n = D.n;
+  A.f2 = f2; A.f3 = f3; A.u_old = u_old; A.u_new = u_new; A.shadow = shadow;
+  A.ca = ca; A.cb = cb; A.cc = cc; A.nterms = nterms;
+  const int n = D.n;
+  if (dir == 0) {
+    const int V = n / 64;
+    if (n % 64 != 0 || !(V == 1 || V == 2 || V == 4 || V == 8))
+      return fail(X3D_ERR_UNSUPPORTED, "fused x sweep needs nx = 64, 128, 256 or 512");
+    const int nlines = ny * nz;
+    const int waves = nlines < 16384 ? nlines : 16384;
+    const dim3 grid((waves + 3) / 4), block(256);
+    switch (V) {
+      case 1: hipLaunchKernelGGL((mom_x_kernel<T, 1>), grid, block, 0, st, A, nlines); break;
+      case 2: hipLaunchKernelGGL((mom_x_kernel<T, 2>), grid, block, 0, st, A, nlines); break;
+      case 4: hipLaunchKernelGGL((mom_x_kernel<T, 4>), grid, block, 0, st, A, nlines); break;
+      default: hipLaunchKernelGGL((mom_x_kernel<T, 8>), grid, block, 0, st, A, nlines); break;
+    }
+  } else {
+    if (nterms) return fail(X3D_ERR_ARG, "AB3 is fused into the x sweep only");
+    if (dir == 1) {
+      A.na = nx; A.nb = nz; A.sa = 1; A.sb = (long long)nx * ny; A.se = nx;
+    } else {
+      A.na = nx; A.nb = ny; A.sa = 1; A.sb = nx; A.se = (long long)nx * ny;
+    }
+    constexpr int L = YzShape<T>::L;
+    YzShape<T> Y;
+    if (!Y.plan(n)) return fail(X3D_ERR_UNSUPPORTED, "fused y/z sweep: line length not supported");
+    if ((size_t)nx * ny * nz * sizeof(T) >= ((size_t)1 << 32))
+      return fail(X3D_ERR_UNSUPPORTED, "fused y/z sweep: field larger than 4 GiB");
+    const double tol = sizeof(T) == 8 ? 1e-18 : 1e-9;
+    A.kdepth = std::max(chain_depth(s1, L, tol), chain_depth(s2, L, tol));
+    const dim3 grid((A.na + Y.TL - 1) / Y.TL, A.nb), block(Y.TL, Y.P);
+    const size_t shmem = Y.shmem;
+    hipLaunchKernelGGL((mom_yz_kernel<T, L>), grid, block, shmem, st, A);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("momentum launch: ") + hipGetErrorString(e));
+  return X3D_OK;
+}
+
+template int launch_momentum<double>(const Context &, int, int, const double *const[3], const double *, double *,
+                                     const double *, const double *, const double *, double *, float *, double,
+                                     double, double, int, int, int, int, hipStream_t);
+template int launch_momentum<float>(const Context &, int, int, const float *const[3], const float *, float *,
+                                    const float *, const float *, const double *, double *, float *, double, double,
+                                    double, int, int, int, int, hipStream_t);
+
+}  // namespace x3d

@@ -11,7 +11,9 @@
 // and product, while the velocity itself is accumulated in FP64 (AB3 and
 // cor_vel) and the Poisson divide and reductions run in FP64.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <utility>
 
 #include "x3d_internal.h"
 
@@ -30,16 +32,24 @@ int launch_tgv_sums(const T *ux, const T *uy, const T *uz, const T *const g[9], 
                     const T *lz, double xnu, size_t n, double *partial, double *out4, hipStream_t st);
 template <typename T>
 int launch_max_mean(const T *a, size_t n, double *partial, double *out2, hipStream_t st);
+bool momentum_fusable(const Context &c);
+template <typename T>
+int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], const T *acc, T *out,
+                    const T *f2, const T *f3, const double *u_old, double *u_new, float *shadow, double ca,
+                    double cb, double cc, int nterms, int nx, int ny, int nz, hipStream_t st);
 
 }  // namespace x3d
 
 struct x3d_solver {
   int precision = 0;
   int itime = 0;
+  bool fused = false;          // fused momentum sweeps (momentum.hip) cover this grid
   size_t n = 0;
   int nx = 0, ny = 0, nz = 0;
-  double *u64[3] = {nullptr, nullptr, nullptr};
-  float *u32[3] = {nullptr, nullptr, nullptr};
+  double *u64[3] = {nullptr, nullptr, nullptr};   // current velocity (FP64)
+  float *u32[3] = {nullptr, nullptr, nullptr};    // its FP32 copy (precision 1)
+  double *u64b[3] = {nullptr, nullptr, nullptr};  // next-step buffers (AB3 writes here,
+  float *u32b[3] = {nullptr, nullptr, nullptr};   // then the two sets swap)
   void *hist[3][3] = {};       // [component][slot], compute precision
   int slot[3] = {0, 1, 2};     // slot index of f1, f2, f3
   void *w[4] = {};             // work fields, compute precision
@@ -70,11 +80,13 @@ int line(const Context &c, int dir, int op, int npaire, const T *u, const T *u2,
   return launch_line<T>(L, c.stream);
 }
 
-// momentum_rhs_eq (transeq.f90:73-470) for one component c of the velocity:
+// momentum_rhs_eq (transeq.f90:73-470) for one component c of the velocity,
+// one line-solve launch per derivative (the path for grids the fused sweeps
+// of momentum.hip do not cover):
 //   out = xnu*(d2x+d2y+d2z) u_c - 1/2 sum_d [ d_d(u_d u_c) + u_d d_d u_c ]
 // accumulated in the reference's order z, y, x.  Parities follow the
-// reference calls: the product and velocity derivative along d use the even
-// (p) coefficient sets when d == c for the product, when d != c otherwise.
+// reference calls: the product derivative along d uses the even (p) sets
+// when d == c, the velocity derivative when d != c.
 template <typename T>
 int momentum_component(const Context &c, const T *const u[3], int comp, T *W, T *out, int nx, int ny, int nz) {
   const double xnu = c.xnu;
@@ -143,13 +155,7 @@ int step_once(Context &c, x3d_solver *S) {
   for (int i = 0; i < 3; ++i) u[i] = sizeof(T) == 8 ? (T *)S->u64[i] : (T *)S->u32[i];
   T *W[4] = {(T *)S->w[0], (T *)S->w[1], (T *)S->w[2], (T *)S->w[3]};
   S->itime += 1;
-  // transeq: new RHS into the f1 slot of each component
-  const T *cu[3] = {u[0], u[1], u[2]};
-  for (int comp = 0; comp < 3; ++comp) {
-    T *f1 = (T *)S->hist[comp][S->slot[0]];
-    X3D_TRY(momentum_component<T>(c, cu, comp, W[0], f1, nx, ny, nz));
-  }
-  // AB3 (time_integrators.f90:85-100)
+  // AB3 coefficients (time_integrators.f90:85-100; variables.f90:1365-1375)
   const double dt = c.p.dt;
   double a, b = 0, cc = 0;
   int nterms;
@@ -160,12 +166,38 @@ int step_once(Context &c, x3d_solver *S) {
   } else {
     a = (23.0 / 12.0) * dt; b = -(16.0 / 12.0) * dt; cc = (5.0 / 12.0) * dt; nterms = 3;
   }
-  for (int comp = 0; comp < 3; ++comp) {
-    const T *f1 = (T *)S->hist[comp][S->slot[0]];
-    const T *f2 = (T *)S->hist[comp][S->slot[1]];
-    const T *f3 = (T *)S->hist[comp][S->slot[2]];
-    X3D_TRY(launch_ab<T>(S->u64[comp], f1, f2, f3, a, b, cc, nterms, sizeof(T) == 8 ? nullptr : S->u32[comp],
-                         S->n, c.stream));
+  // transeq (z, y, x sweeps per component) with AB3 fused into the x sweep;
+  // the new velocity goes to the second buffer set
+  const T *cu[3] = {u[0], u[1], u[2]};
+  if (S->fused) {
+    for (int comp = 0; comp < 3; ++comp) {
+      T *f1 = (T *)S->hist[comp][S->slot[0]];
+      const T *f2 = (T *)S->hist[comp][S->slot[1]];
+      const T *f3 = (T *)S->hist[comp][S->slot[2]];
+      X3D_TRY(launch_momentum<T>(c, 2, comp, cu, nullptr, W[0], nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0,
+                                 0, 0, nx, ny, nz, c.stream));
+      X3D_TRY(launch_momentum<T>(c, 1, comp, cu, W[0], W[0], nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0,
+                                 0, nx, ny, nz, c.stream));
+      X3D_TRY(launch_momentum<T>(c, 0, comp, cu, W[0], f1, f2, f3, S->u64[comp], S->u64b[comp],
+                                 sizeof(T) == 8 ? nullptr : S->u32b[comp], a, b, cc, nterms, nx, ny, nz,
+                                 c.stream));
+    }
+    for (int i = 0; i < 3; ++i) {
+      std::swap(S->u64[i], S->u64b[i]);
+      std::swap(S->u32[i], S->u32b[i]);
+      u[i] = sizeof(T) == 8 ? (T *)S->u64[i] : (T *)S->u32[i];
+      cu[i] = u[i];
+    }
+  } else {
+    for (int comp = 0; comp < 3; ++comp)
+      X3D_TRY(momentum_component<T>(c, cu, comp, W[0], (T *)S->hist[comp][S->slot[0]], nx, ny, nz));
+    for (int comp = 0; comp < 3; ++comp) {
+      const T *f1 = (T *)S->hist[comp][S->slot[0]];
+      const T *f2 = (T *)S->hist[comp][S->slot[1]];
+      const T *f3 = (T *)S->hist[comp][S->slot[2]];
+      X3D_TRY(launch_ab<T>(S->u64[comp], f1, f2, f3, a, b, cc, nterms, sizeof(T) == 8 ? nullptr : S->u32[comp],
+                           S->n, c.stream));
+    }
   }
   // history rotation (f3 <- f2 <- f1) by slot permutation
   const int s1 = S->slot[0], s2 = S->slot[1], s3 = S->slot[2];
@@ -223,6 +255,10 @@ int alloc_solver(Context &c, x3d_solver *S) {
   for (int i = 0; i < 3; ++i) {
     X3D_HIP(hipMalloc(&S->u64[i], n * 8));
     if (S->precision == 1) X3D_HIP(hipMalloc(&S->u32[i], n * 4));
+    if (S->fused) {
+      X3D_HIP(hipMalloc(&S->u64b[i], n * 8));
+      if (S->precision == 1) X3D_HIP(hipMalloc(&S->u32b[i], n * 4));
+    }
     for (int j = 0; j < 3; ++j) {
       X3D_HIP(hipMalloc(&S->hist[i][j], n * es));
       X3D_HIP(hipMemsetAsync(S->hist[i][j], 0, n * es, c.stream));
@@ -239,6 +275,8 @@ void free_solver(x3d_solver *S) {
   for (int i = 0; i < 3; ++i) {
     if (S->u64[i]) (void)hipFree(S->u64[i]);
     if (S->u32[i]) (void)hipFree(S->u32[i]);
+    if (S->u64b[i]) (void)hipFree(S->u64b[i]);
+    if (S->u32b[i]) (void)hipFree(S->u32b[i]);
     for (int j = 0; j < 3; ++j)
       if (S->hist[i][j]) (void)hipFree(S->hist[i][j]);
   }
@@ -289,6 +327,7 @@ int x3d_solver_create(int precision, x3d_solver **out) {
   S->ny = c->p.ny;
   S->nz = c->p.nz;
   S->n = (size_t)S->nx * S->ny * S->nz;
+  S->fused = momentum_fusable(*c) && getenv("X3D_UNFUSED_MOMENTUM") == nullptr;
   int r = alloc_solver(*c, S);
   if (r != X3D_OK) {
     free_solver(S);
