@@ -10,14 +10,19 @@
 // (nx/2+1, :, nz/2+1) line zeroed.
 //
 // Passes over the half spectrum cw (nxh = nx/2+1, ny, nz), kx fastest:
-//   1 r2c along x (real line packed as nx/2 complex points, LDS FFT, unpack)
+//   1 r2c along x (real line packed as nx/2 complex points, FFT, unpack)
 //   2 c2c forward along y            (DCT-II for a non-periodic y)
 //   3 c2c forward along z, divide by -K (FP64), c2c inverse along z
 //   4 c2c inverse along y            (DCT-III for a non-periodic y)
 //   5 c2r along x
-// Every line FFT is a radix-4 (+radix-2) Stockham transform held in LDS;
-// blocks hold N*TL = 4096 complex points so that strided y/z lines are read
-// and written as rows of TL consecutive kx.
+// Line FFTs run in registers: each thread holds E = 16 points of a line in
+// cyclic order (point t + m*NT for thread t of NT), does radix-16 (then
+// radix-2/4/8) Stockham butterflies on them and exchanges through LDS only
+// between stages -- one exchange for lines of up to 256 points.  The
+// first stage reads its inputs straight from HBM and the last stage leaves
+// the outputs in the same cyclic order, so they are stored straight back;
+// the DCT reorders, the Poisson divide and the real-to-complex packing are
+// folded into those loads and stores.
 #include <cmath>
 
 #include "x3d_internal.h"
@@ -25,7 +30,7 @@
 namespace x3d {
 
 template <typename T>
-struct cplx {
+struct alignas(2 * sizeof(T)) cplx {
   T x, y;
 };
 template <typename T>
@@ -38,105 +43,6 @@ template <typename T>
 __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
 template <typename T>
 __device__ __forceinline__ cplx<T> cconj(cplx<T> a) { return {a.x, -a.y}; }
-
-constexpr int FFT_BLOCK_POINTS = 4096;
-
-__device__ __forceinline__ int ilog2_dev(int n) { return 31 - __clz(n); }
-constexpr int FFT_THREADS = 256;
-
-// In-place Stockham FFT of TL lines of length N held in LDS as buf[e*TLp + l].
-// tw[m] = exp(-2 pi i m / N); inverse uses the conjugate.
-template <typename T>
-__device__ void lds_fft(cplx<T> *buf, int N, int lg, int TL, int TLp, const cplx<T> *__restrict__ tw,
-                        bool inverse) {
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  int Ns = 1;
-  const int nr4 = lg >> 1;
-  for (int st = 0; st < nr4; ++st) {
-    const int quarter = N >> 2;
-    const int nb = quarter * TL;
-    cplx<T> v[4][4];
-    int jj[4], ll[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int bidx = tid + q * nthr;
-      jj[q] = -1;
-      if (bidx < nb) {
-        const int l = bidx % TL, j = bidx / TL;
-        jj[q] = j;
-        ll[q] = l;
-        const int k = j % Ns;
-        const int step = N / (Ns * 4);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          cplx<T> x = buf[(j + r * quarter) * TLp + l];
-          if (r > 0) {
-            cplx<T> w = tw[r * k * step];
-            if (inverse) w.y = -w.y;
-            x = cmul(x, w);
-          }
-          v[q][r] = x;
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (jj[q] >= 0) {
-        const cplx<T> a = cadd(v[q][0], v[q][2]);
-        const cplx<T> b = csub(v[q][0], v[q][2]);
-        const cplx<T> c = cadd(v[q][1], v[q][3]);
-        const cplx<T> dd = csub(v[q][1], v[q][3]);
-        // forward: d*(-i); inverse: d*(+i)
-        const cplx<T> d = inverse ? cplx<T>{-dd.y, dd.x} : cplx<T>{dd.y, -dd.x};
-        const int j = jj[q], l = ll[q];
-        const int k = j % Ns;
-        const int o = (j / Ns) * Ns * 4 + k;
-        buf[(o)*TLp + l] = cadd(a, c);
-        buf[(o + Ns) * TLp + l] = cadd(b, d);
-        buf[(o + 2 * Ns) * TLp + l] = csub(a, c);
-        buf[(o + 3 * Ns) * TLp + l] = csub(b, d);
-      }
-    }
-    __syncthreads();
-    Ns *= 4;
-  }
-  if (lg & 1) {
-    const int half = N >> 1;
-    const int nb = half * TL;
-    cplx<T> v[8][2];
-    int jj[8], ll[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int bidx = tid + q * nthr;
-      jj[q] = -1;
-      if (bidx < nb) {
-        const int l = bidx % TL, j = bidx / TL;
-        jj[q] = j;
-        ll[q] = l;
-        const int k = j % Ns;
-        const int step = N / (Ns * 2);
-        v[q][0] = buf[j * TLp + l];
-        cplx<T> x = buf[(j + half) * TLp + l];
-        cplx<T> w = tw[k * step];
-        if (inverse) w.y = -w.y;
-        v[q][1] = cmul(x, w);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      if (jj[q] >= 0) {
-        const int j = jj[q], l = ll[q];
-        const int k = j % Ns;
-        const int o = (j / Ns) * Ns * 2 + k;
-        buf[o * TLp + l] = cadd(v[q][0], v[q][1]);
-        buf[(o + Ns) * TLp + l] = csub(v[q][0], v[q][1]);
-      }
-    }
-    __syncthreads();
-  }
-}
 
 template <typename T>
 struct PoissonTables {
@@ -164,183 +70,388 @@ static int ilog2(int n) {
   return (1 << l) == n ? l : -1;
 }
 
-// ---------------------------------------------------------------- pass 1: r2c along x
+// ---------------------------------------------------------------- radix-R DFTs in registers
+// Natural order in and out; forward uses exp(-2 pi i/R), inverse exp(+2 pi i/R).
+template <typename T, bool INV>
+__device__ __forceinline__ cplx<T> rot_mi(cplx<T> a) {   // a*(-i) forward, a*(+i) inverse
+  return INV ? cplx<T>{-a.y, a.x} : cplx<T>{a.y, -a.x};
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(cplx<T> &x0, cplx<T> &x1, cplx<T> &x2, cplx<T> &x3) {
+  const cplx<T> a = cadd(x0, x2), b = csub(x0, x2), c = cadd(x1, x3), d = rot_mi<T, INV>(csub(x1, x3));
+  x0 = cadd(a, c);
+  x1 = cadd(b, d);
+  x2 = csub(a, c);
+  x3 = csub(b, d);
+}
+
+// exp(-+2 pi i m/16) for the constant twiddles of the radix-8/16 kernels
+template <typename T, bool INV, int M>
+__device__ __forceinline__ cplx<T> w16() {
+  constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+  constexpr int m = M & 15;
+  constexpr double re = m == 0 ? 1.0 : m == 1 ? c1 : m == 2 ? h : m == 3 ? s1 : m == 4 ? 0.0 : m == 5 ? -s1
+                      : m == 6 ? -h : m == 7 ? -c1 : m == 8 ? -1.0 : m == 9 ? -c1 : m == 10 ? -h : m == 11 ? -s1
+                      : m == 12 ? 0.0 : m == 13 ? s1 : m == 14 ? h : c1;
+  constexpr double im = m == 0 ? 0.0 : m == 1 ? -s1 : m == 2 ? -h : m == 3 ? -c1 : m == 4 ? -1.0 : m == 5 ? -c1
+                      : m == 6 ? -h : m == 7 ? -s1 : m == 8 ? 0.0 : m == 9 ? s1 : m == 10 ? h : m == 11 ? c1
+                      : m == 12 ? 1.0 : m == 13 ? c1 : m == 14 ? h : s1;
+  return cplx<T>{(T)re, (T)(INV ? -im : im)};
+}
+
+template <typename T, int R, bool INV>
+struct Dft;
+
+template <typename T, bool INV>
+struct Dft<T, 1, INV> {
+  static __device__ __forceinline__ void run(cplx<T> (&)[1]) {}
+};
+template <typename T, bool INV>
+struct Dft<T, 2, INV> {
+  static __device__ __forceinline__ void run(cplx<T> (&v)[2]) {
+    const cplx<T> a = cadd(v[0], v[1]), b = csub(v[0], v[1]);
+    v[0] = a;
+    v[1] = b;
+  }
+};
+template <typename T, bool INV>
+struct Dft<T, 4, INV> {
+  static __device__ __forceinline__ void run(cplx<T> (&v)[4]) { dft4<T, INV>(v[0], v[1], v[2], v[3]); }
+};
+// 8 = 4 x 2: r = 2a + b, k = k1 + 4 k2
+template <typename T, bool INV>
+struct Dft<T, 8, INV> {
+  static __device__ __forceinline__ void run(cplx<T> (&v)[8]) {
+    dft4<T, INV>(v[0], v[2], v[4], v[6]);
+    dft4<T, INV>(v[1], v[3], v[5], v[7]);
+    v[3] = cmul(v[3], w16<T, INV, 2>());
+    v[5] = rot_mi<T, INV>(v[5]);
+    v[7] = cmul(v[7], w16<T, INV, 6>());
+    cplx<T> o[8];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      o[k1] = cadd(v[2 * k1], v[2 * k1 + 1]);
+      o[k1 + 4] = csub(v[2 * k1], v[2 * k1 + 1]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = o[k];
+  }
+};
+// 16 = 4 x 4: r = 4a + b, k = k1 + 4 k2
+template <typename T, bool INV>
+struct Dft<T, 16, INV> {
+  static __device__ __forceinline__ void run(cplx<T> (&v)[16]) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) dft4<T, INV>(v[b], v[4 + b], v[8 + b], v[12 + b]);
+    // Y_b[k1] sits in v[4 k1 + b]; twiddle by w16^(b k1)
+    v[5] = cmul(v[5], w16<T, INV, 1>());
+    v[6] = cmul(v[6], w16<T, INV, 2>());
+    v[7] = cmul(v[7], w16<T, INV, 3>());
+    v[9] = cmul(v[9], w16<T, INV, 2>());
+    v[10] = rot_mi<T, INV>(v[10]);
+    v[11] = cmul(v[11], w16<T, INV, 6>());
+    v[13] = cmul(v[13], w16<T, INV, 3>());
+    v[14] = cmul(v[14], w16<T, INV, 6>());
+    v[15] = cmul(v[15], w16<T, INV, 9>());
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) dft4<T, INV>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+    // X[k1 + 4 k2] sits in v[4 k1 + k2]
+    cplx<T> o[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) o[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = o[k];
+  }
+};
+
+// ---------------------------------------------------------------- line FFT
+// Points per thread and threads per line for a line of N points.
+template <int N>
+struct LineShape {
+  static constexpr int E = N >= 16 ? 16 : N;
+  static constexpr int NT = N / E;
+};
+
+// Stockham stages on a line held in cyclic order: thread t owns points
+// t + m*NT (m < E).  A stage of radix R does E/R butterflies per thread,
+// j = t + q*NT, whose inputs are the thread's points q + r*E/R.  Its outputs
+// go to (j/Ns)*Ns*R + j%Ns + r*Ns; for the last stage (Ns*R = N) that is the
+// input slot again, otherwise they go through LDS and are read back in
+// cyclic order.  tw[m] = exp(-2 pi i m/N) in LDS.  LD maps a point index to
+// the thread's LDS slot.  Every thread of the block calls this.
+template <typename T, int N, bool INV, int NS, typename LD>
+__device__ __forceinline__ void fft_stages(cplx<T> (&x)[LineShape<N>::E], const int t, const LD &ld,
+                                           const cplx<T> *tw) {
+  constexpr int E = LineShape<N>::E, NT = LineShape<N>::NT;
+  constexpr int R = (N / NS) < E ? (N / NS) : E;
+  constexpr int Q = E / R;
+  constexpr bool LAST = NS * R == N;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = t + q * NT;
+    const int k = j & (NS - 1);
+    cplx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = x[q + r * Q];
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cplx<T> w = tw[r * k * (N / (NS * R))];
+        if (INV) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    Dft<T, R, INV>::run(v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[q + r * Q] = v[r];
+  }
+  if constexpr (!LAST) {
+    __syncthreads();   // the previous use of the exchange buffer is over
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = t + q * NT;
+      const int k = j & (NS - 1);
+      const int o = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) ld.store(o + r * NS, x[q + r * Q]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) x[m] = ld.load(t + m * NT);
+    fft_stages<T, N, INV, NS * R, LD>(x, t, ld, tw);
+  }
+}
+
+// LDS slot maps.  Strided (y/z) blocks: point-major rows of TL lines, lanes
+// run over lines.  x blocks: line-major, lanes run over points; one pad slot
+// per 16 points keeps the radix-16 scatter off a single bank.
 template <typename T>
+struct LdRows {
+  cplx<T> *buf;
+  int TL, l;
+  __device__ __forceinline__ void store(int p, cplx<T> v) const { buf[p * TL + l] = v; }
+  __device__ __forceinline__ cplx<T> load(int p) const { return buf[p * TL + l]; }
+};
+template <typename T>
+struct LdLine {
+  cplx<T> *buf;
+  int base;
+  __device__ __forceinline__ void store(int p, cplx<T> v) const { buf[base + p + (p >> 4)] = v; }
+  __device__ __forceinline__ cplx<T> load(int p) const { return buf[base + p + (p >> 4)]; }
+};
+
+template <typename T>
+__device__ __forceinline__ void stage_table(cplx<T> *dst, const cplx<T> *__restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+constexpr int FFT_THREADS = 256;
+
+// x blocks: XT lines of M = nx/2 complex points, NT threads per line.
+template <int M>
+struct XShape {
+  static constexpr int NT = LineShape<M>::NT;
+  static constexpr int TL = FFT_THREADS / NT;
+  static constexpr int LP = M + M / 16;             // padded line length in LDS
+  template <typename T>
+  static constexpr size_t lds() { return sizeof(cplx<T>) * ((size_t)TL * LP + 2 * M + 1); }
+};
+// strided blocks: TL kx x N points, NT threads per line.
+template <int N>
+struct SShape {
+  static constexpr int NT = LineShape<N>::NT;
+  static constexpr int TL = (FFT_THREADS / NT) < 64 ? (FFT_THREADS / NT) : 64;
+  template <typename T>
+  static constexpr size_t lds() { return sizeof(cplx<T>) * ((size_t)TL * N + 2 * N); }
+};
+
+// ---------------------------------------------------------------- pass 1: r2c along x
+// X_k = (Z_k + conj Z_{M-k})/2 - i/2 e^{-2 pi i k/nx} (Z_k - conj Z_{M-k}), k = 0..M
+template <typename T, int M>
 __global__ void __launch_bounds__(FFT_THREADS) r2c_x_kernel(const T *__restrict__ in, cplx<T> *__restrict__ out,
-                                                             int nx, int nlines, PoissonTables<T> tb) {
+                                                             int nlines, PoissonTables<T> tb) {
+  using S = XShape<M>;
+  constexpr int E = LineShape<M>::E, NT = S::NT, TL = S::TL;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-  const int M = nx >> 1, lg = ilog2_dev(M);
-  const int TL = FFT_BLOCK_POINTS / M, TLp = TL + 1;
-  const int l0 = blockIdx.x * TL;
-  const int nv = min(TL, nlines - l0);
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const T *src = in + (long long)l0 * nx;
-  for (int idx = tid; idx < nv * M; idx += nthr) {
-    const int l = idx / M, m = idx - l * M;
-    const T *p = src + (long long)l * nx + 2 * m;
-    buf[m * TLp + l] = cplx<T>{p[0], p[1]};
-  }
-  for (int idx = nv * M + tid; idx < TL * M; idx += nthr) {
-    const int l = idx / M, m = idx - l * M;
-    buf[m * TLp + l] = cplx<T>{T(0), T(0)};
-  }
+  cplx<T> *tw = buf + TL * S::LP, *rt = tw + M;
+  const int t = threadIdx.x % NT, l = threadIdx.x / NT;
+  const int line = blockIdx.x * TL + l;
+  const bool valid = line < nlines;
+  stage_table(tw, tb.twx, M);
+  stage_table(rt, tb.rtx, M + 1);
+  cplx<T> x[E];
+  const cplx<T> *src = reinterpret_cast<const cplx<T> *>(in + (size_t)line * 2 * M);
+#pragma unroll
+  for (int m = 0; m < E; ++m) x[m] = valid ? src[t + m * NT] : cplx<T>{T(0), T(0)};
   __syncthreads();
-  lds_fft<T>(buf, M, lg, TL, TLp, tb.twx, false);
-  // unpack: X_k = (Z_k + conj Z_{M-k})/2 - i/2 e^{-2 pi i k/nx} (Z_k - conj Z_{M-k})
-  const int nxh = M + 1;
-  cplx<T> *dst = out + (long long)l0 * nxh;
-  for (int idx = tid; idx < nv * nxh; idx += nthr) {
-    const int l = idx / nxh, k = idx - l * nxh;
-    const cplx<T> zk = buf[(k == M ? 0 : k) * TLp + l];
-    const cplx<T> zm = cconj(buf[(k == 0 ? 0 : M - k) * TLp + l]);
-    const cplx<T> e = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y + zm.y)};
-    const cplx<T> o0 = {T(0.5) * (zk.x - zm.x), T(0.5) * (zk.y - zm.y)};
-    const cplx<T> o = {o0.y, -o0.x};  // (Z_k - conj Z_{M-k}) / (2i)
-    dst[(long long)l * nxh + k] = cadd(e, cmul(tb.rtx[k], o));
+  const LdLine<T> ld{buf, l * S::LP};
+  fft_stages<T, M, false, 1>(x, t, ld, tw);
+  // Z_{M-k} belongs to another thread: exchange
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) ld.store(t + m * NT, x[m]);
+  __syncthreads();
+  if (valid) {
+    cplx<T> *dst = out + (size_t)line * (M + 1);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = t + m * NT;
+      const cplx<T> zk = x[m];
+      const cplx<T> zm = cconj(ld.load(k == 0 ? 0 : M - k));
+      const cplx<T> e = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y + zm.y)};
+      const cplx<T> o = {T(0.5) * (zk.y - zm.y), -T(0.5) * (zk.x - zm.x)};   // (Z_k - conj Z_{M-k}) / (2i)
+      dst[k] = cadd(e, cmul(rt[k], o));
+      if (k == 0) {   // k = M uses Z_0 on both sides
+        dst[M] = cadd(e, cmul(rt[M], o));
+      }
+    }
   }
 }
 
 // ---------------------------------------------------------------- pass 5: c2r along x
-template <typename T>
+// Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/nx} (X_k - conj X_{M-k}),  k < M
+template <typename T, int M>
 __global__ void __launch_bounds__(FFT_THREADS) c2r_x_kernel(const cplx<T> *__restrict__ in, T *__restrict__ out,
-                                                             int nx, int nlines, PoissonTables<T> tb) {
+                                                             int nlines, PoissonTables<T> tb) {
+  using S = XShape<M>;
+  constexpr int E = LineShape<M>::E, NT = S::NT, TL = S::TL;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-  const int M = nx >> 1, lg = ilog2_dev(M), nxh = M + 1;
-  const int TL = FFT_BLOCK_POINTS / M, TLp = TL + 1;
-  const int l0 = blockIdx.x * TL;
-  const int nv = min(TL, nlines - l0);
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const cplx<T> *src = in + (long long)l0 * nxh;
-  // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/nx} (X_k - conj X_{M-k}),  k < M
-  for (int idx = tid; idx < TL * M; idx += nthr) {
-    const int l = idx / M, k = idx - l * M;
-    cplx<T> z = {T(0), T(0)};
-    if (l < nv) {
-      const cplx<T> xk = src[(long long)l * nxh + k];
-      const cplx<T> xm = cconj(src[(long long)l * nxh + (M - k)]);
-      const cplx<T> e = cadd(xk, xm);
-      const cplx<T> d = csub(xk, xm);
-      const cplx<T> w = cconj(tb.rtx[k]);
-      const cplx<T> od = cmul(w, d);
-      z = cplx<T>{e.x - od.y, e.y + od.x};
-    }
-    buf[k * TLp + l] = z;
+  cplx<T> *tw = buf + TL * S::LP, *rt = tw + M;
+  const int t = threadIdx.x % NT, l = threadIdx.x / NT;
+  const int line = blockIdx.x * TL + l;
+  const bool valid = line < nlines;
+  stage_table(tw, tb.twx, M);
+  stage_table(rt, tb.rtx, M + 1);
+  const cplx<T> *src = in + (size_t)line * (M + 1);
+  cplx<T> xk[E], xm[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + m * NT;
+    xk[m] = valid ? src[k] : cplx<T>{T(0), T(0)};
+    xm[m] = valid ? src[M - k] : cplx<T>{T(0), T(0)};
   }
   __syncthreads();
-  lds_fft<T>(buf, M, lg, TL, TLp, tb.twx, true);
-  T *dst = out + (long long)l0 * nx;
-  for (int idx = tid; idx < nv * M; idx += nthr) {
-    const int l = idx / M, m = idx - l * M;
-    const cplx<T> z = buf[m * TLp + l];
-    T *p = dst + (long long)l * nx + 2 * m;
-    p[0] = z.x;
-    p[1] = z.y;
+  cplx<T> x[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + m * NT;
+    const cplx<T> c = cconj(xm[m]);
+    const cplx<T> e = cadd(xk[m], c);
+    const cplx<T> od = cmul(cconj(rt[k]), csub(xk[m], c));
+    x[m] = cplx<T>{e.x - od.y, e.y + od.x};
+  }
+  const LdLine<T> ld{buf, l * S::LP};
+  fft_stages<T, M, true, 1>(x, t, ld, tw);
+  if (valid) {
+    cplx<T> *dst = reinterpret_cast<cplx<T> *>(out + (size_t)line * 2 * M);
+#pragma unroll
+    for (int m = 0; m < E; ++m) dst[t + m * NT] = x[m];
   }
 }
 
 // ---------------------------------------------------------------- passes 2-4: strided lines
 enum { FFT_Y_FWD = 0, FFT_Y_INV = 1, FFT_Z_SOLVE = 2 };
 
-template <typename T, int MODE>
+// row of point p under the even/odd reorder of poisson.f90:688-697 / :997-1008
+__device__ __forceinline__ int dct_row(int p, int N) { return p < (N >> 1) ? 2 * p : 2 * (N - 1 - p) + 1; }
+
+template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(FFT_THREADS) strided_kernel(cplx<T> *__restrict__ cw, int nxh, int ny, int nz,
                                                                int ydct, double scale, int zero_line,
                                                                PoissonTables<T> tb) {
+  using S = SShape<N>;
+  constexpr int E = LineShape<N>::E, NT = S::NT, TL = S::TL;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *buf = reinterpret_cast<cplx<T> *>(smem_raw);
-  const int N = MODE == FFT_Z_SOLVE ? nz : ny;
-  const int lg = ilog2_dev(N);
-  const int TL = FFT_BLOCK_POINTS / N, TLp = TL + 1;
-  const int a0 = blockIdx.x * TL;            // kx tile
+  cplx<T> *tw = buf + TL * N, *dct = tw + N;
+  const int l = threadIdx.x % TL, t = threadIdx.x / TL;
+  const int a = blockIdx.x * TL + l;         // kx
   const int b = blockIdx.y;                  // kz (y passes) or ky (z pass)
-  const int nv = min(TL, nxh - a0);
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const long long se = MODE == FFT_Z_SOLVE ? (long long)nxh * ny : (long long)nxh;
-  const long long base = MODE == FFT_Z_SOLVE ? (long long)b * nxh + a0 : (long long)b * nxh * ny + a0;
-  const cplx<T> *tw = MODE == FFT_Z_SOLVE ? tb.twz : tb.twy;
-
-  // ---- load (DCT forward: even/odd reorder, poisson.f90:688-697)
-  for (int idx = tid; idx < TL * N; idx += nthr) {
-    const int e = idx / TL, l = idx - e * TL;
-    cplx<T> x = {T(0), T(0)};
-    if (l < nv) x = cw[base + (long long)e * se + l];
-    int pos = e;
-    if (MODE == FFT_Y_FWD && ydct) pos = (e & 1) ? N - 1 - (e >> 1) : (e >> 1);
-    buf[pos * TLp + l] = x;
+  const bool valid = a < nxh;
+  const size_t se = MODE == FFT_Z_SOLVE ? (size_t)nxh * ny : (size_t)nxh;
+  const size_t base = MODE == FFT_Z_SOLVE ? (size_t)b * nxh + a : (size_t)b * nxh * ny + a;
+  stage_table(tw, MODE == FFT_Z_SOLVE ? tb.twz : tb.twy, N);
+  if (MODE != FFT_Z_SOLVE && ydct) stage_table(dct, tb.dcty, N);
+  const bool reorder_in = MODE == FFT_Y_FWD && ydct;
+  cplx<T> x[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int p = t + m * NT;
+    const int row = reorder_in ? dct_row(p, N) : p;
+    x[m] = valid ? cw[base + (size_t)row * se] : cplx<T>{T(0), T(0)};
   }
-  __syncthreads();
-
-  if (MODE == FFT_Y_INV && ydct) {
-    // DCT-III input side: V_k = e^{+i pi k/2N} (C_k - i C_{N-k}), V_0 = C_0
-    // (poisson.f90:924-943).  Each thread owns the pair (k, N-k).
-    const int half = N >> 1;
-    for (int bidx = tid; bidx < (half + 1) * TL; bidx += nthr) {
-      const int l = bidx % TL, k = bidx / TL;
-      const cplx<T> ck = buf[k * TLp + l];
-      const cplx<T> cm = (k == 0) ? cplx<T>{T(0), T(0)} : buf[(N - k) * TLp + l];
-      const cplx<T> vk = cmul(cconj(tb.dcty[k]), cplx<T>{ck.x + cm.y, ck.y - cm.x});
-      if (k > 0 && k < half) {
-        const cplx<T> vm = cmul(cconj(tb.dcty[N - k]), cplx<T>{cm.x + ck.y, cm.y - ck.x});
-        buf[(N - k) * TLp + l] = vm;
+  if constexpr (MODE == FFT_Y_INV) {
+    if (ydct) {
+      // DCT-III input side: V_k = e^{+i pi k/2N} (C_k - i C_{N-k}), V_0 = C_0
+      // (poisson.f90:924-943); C_{N-k} is read from its own row
+      cplx<T> cm[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + m * NT;
+        cm[m] = (valid && k > 0) ? cw[base + (size_t)(N - k) * se] : cplx<T>{T(0), T(0)};
       }
-      buf[k * TLp + l] = vk;
+      __syncthreads();   // dct table
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + m * NT;
+        x[m] = cmul(cconj(dct[k]), cplx<T>{x[m].x + cm[m].y, x[m].y - cm[m].x});
+      }
     }
-    __syncthreads();
   }
-
-  lds_fft<T>(buf, N, lg, TL, TLp, tw, MODE == FFT_Y_INV);
+  __syncthreads();   // tables staged
+  const LdRows<T> ld{buf, TL, l};
+  fft_stages<T, N, MODE == FFT_Y_INV, 1>(x, t, ld, tw);
 
   if (MODE == FFT_Y_FWD && ydct) {
-    // DCT-II output side: C_k = (e^{-i pi k/2N} V_k + e^{+i pi k/2N} V_{N-k}) / 2
-    // (poisson.f90:761-781).  Each thread owns the pair (k, N-k).
-    const int half = N >> 1;
-    for (int bidx = tid; bidx < (half + 1) * TL; bidx += nthr) {
-      const int l = bidx % TL, k = bidx / TL;
-      const cplx<T> vk = buf[k * TLp + l];
-      if (k == 0) continue;
-      const cplx<T> vm = buf[(N - k) * TLp + l];
-      const cplx<T> a = cmul(tb.dcty[k], vk);
-      const cplx<T> bb = cmul(cconj(tb.dcty[k]), vm);
-      if (k < half) {
-        const cplx<T> a2 = cmul(tb.dcty[N - k], vm);
-        const cplx<T> b2 = cmul(cconj(tb.dcty[N - k]), vk);
-        buf[(N - k) * TLp + l] = cplx<T>{T(0.5) * (a2.x + b2.x), T(0.5) * (a2.y + b2.y)};
-      }
-      buf[k * TLp + l] = cplx<T>{T(0.5) * (a.x + bb.x), T(0.5) * (a.y + bb.y)};
-    }
+    // DCT-II output side: C_k = (e^{-i pi k/2N} V_k + e^{+i pi k/2N} V_{N-k}) / 2,
+    // C_0 = V_0 (poisson.f90:761-781); V_{N-k} belongs to another thread
     __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) ld.store(t + m * NT, x[m]);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = t + m * NT;
+      if (k > 0) {
+        const cplx<T> vm = ld.load(N - k);
+        const cplx<T> p = cmul(dct[k], x[m]);
+        const cplx<T> q = cmul(cconj(dct[k]), vm);
+        x[m] = cplx<T>{T(0.5) * (p.x + q.x), T(0.5) * (p.y + q.y)};
+      }
+    }
   }
 
   if (MODE == FFT_Z_SOLVE) {
     // divide by -K in FP64 (poisson.f90:362-371 / :795-819)
-    const int ky = b;
-    const double ky2 = tb.ky2[ky], tyv = tb.ty[ky];
-    for (int idx = tid; idx < TL * N; idx += nthr) {
-      const int kz = idx / TL, l = idx - kz * TL;
-      const int kx = a0 + l;
-      cplx<T> x = buf[kz * TLp + l];
-      if (kx < nxh) {
-        const double txv = tb.tx[kx], tzv = tb.tz[kz];
+    if (valid) {
+      const double ky2 = tb.ky2[b], tyv = tb.ty[b];
+      const double txv = tb.tx[a], kx2 = tb.kx2[a];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int kz = t + m * NT;
+        const double tzv = tb.tz[kz];
         const double yz = tyv * tzv, xz = txv * tzv, xy = txv * tyv;
-        const double K = tb.kx2[kx] * (yz * yz) + ky2 * (xz * xz) + tb.kz2[kz] * (xy * xy);
-        const bool zero = (K < 1.0e-16) || (zero_line && kx == (nxh - 1) && kz == (N >> 1));
+        const double K = kx2 * (yz * yz) + ky2 * (xz * xz) + tb.kz2[kz] * (xy * xy);
+        const bool zero = (K < 1.0e-16) || (zero_line && a == (nxh - 1) && kz == (N >> 1));
         const double f = zero ? 0.0 : -scale / K;
-        x = cplx<T>{(T)((double)x.x * f), (T)((double)x.y * f)};
+        x[m] = cplx<T>{(T)((double)x[m].x * f), (T)((double)x[m].y * f)};
       }
-      buf[kz * TLp + l] = x;
     }
-    __syncthreads();
-    lds_fft<T>(buf, N, lg, TL, TLp, tw, true);
+    fft_stages<T, N, true, 1>(x, t, ld, tw);
   }
 
-  // ---- store (DCT inverse: undo the even/odd reorder, poisson.f90:997-1008)
-  for (int idx = tid; idx < TL * N; idx += nthr) {
-    const int e = idx / TL, l = idx - e * TL;
-    if (l < nv) {
-      int pos = e;
-      if (MODE == FFT_Y_INV && ydct) pos = (e & 1) ? N - 1 - (e >> 1) : (e >> 1);
-      cw[base + (long long)e * se + l] = buf[pos * TLp + l];
+  if (valid) {
+    const bool reorder_out = MODE == FFT_Y_INV && ydct;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int p = t + m * NT;
+      const int row = reorder_out ? dct_row(p, N) : p;
+      cw[base + (size_t)row * se] = x[m];
     }
   }
 }
@@ -512,6 +623,54 @@ void poisson_free(Context &c) {
   c.poisson = nullptr;
 }
 
+template <typename T, int M>
+static void launch_x(bool fwd, T *rhs, cplx<T> *cw, int nlines, const PoissonTables<T> &tb, hipStream_t st) {
+  using S = XShape<M>;
+  const dim3 grid((nlines + S::TL - 1) / S::TL), block(S::TL * S::NT);
+  const size_t sh = S::template lds<T>();
+  if (fwd) hipLaunchKernelGGL((r2c_x_kernel<T, M>), grid, block, sh, st, rhs, cw, nlines, tb);
+  else hipLaunchKernelGGL((c2r_x_kernel<T, M>), grid, block, sh, st, cw, rhs, nlines, tb);
+}
+
+template <typename T, int N, int MODE>
+static void launch_s(const PoissonPlan &P, cplx<T> *cw, const PoissonTables<T> &tb, hipStream_t st) {
+  using S = SShape<N>;
+  const dim3 grid((P.nxh + S::TL - 1) / S::TL, MODE == FFT_Z_SOLVE ? P.ny : P.nz), block(S::TL * S::NT);
+  hipLaunchKernelGGL((strided_kernel<T, N, MODE>), grid, block, S::template lds<T>(), st, cw, P.nxh, P.ny, P.nz,
+                     P.ydct, P.scale, MODE == FFT_Z_SOLVE ? P.ydct : 0, tb);
+}
+
+template <typename T, int MODE>
+static bool dispatch_s(int N, const PoissonPlan &P, cplx<T> *cw, const PoissonTables<T> &tb, hipStream_t st) {
+  switch (N) {
+    case 8: launch_s<T, 8, MODE>(P, cw, tb, st); return true;
+    case 16: launch_s<T, 16, MODE>(P, cw, tb, st); return true;
+    case 32: launch_s<T, 32, MODE>(P, cw, tb, st); return true;
+    case 64: launch_s<T, 64, MODE>(P, cw, tb, st); return true;
+    case 128: launch_s<T, 128, MODE>(P, cw, tb, st); return true;
+    case 256: launch_s<T, 256, MODE>(P, cw, tb, st); return true;
+    case 512: launch_s<T, 512, MODE>(P, cw, tb, st); return true;
+    case 1024: launch_s<T, 1024, MODE>(P, cw, tb, st); return true;
+    default: return false;
+  }
+}
+
+template <typename T>
+static bool dispatch_x(bool fwd, int M, T *rhs, cplx<T> *cw, int nlines, const PoissonTables<T> &tb,
+                       hipStream_t st) {
+  switch (M) {
+    case 8: launch_x<T, 8>(fwd, rhs, cw, nlines, tb, st); return true;
+    case 16: launch_x<T, 16>(fwd, rhs, cw, nlines, tb, st); return true;
+    case 32: launch_x<T, 32>(fwd, rhs, cw, nlines, tb, st); return true;
+    case 64: launch_x<T, 64>(fwd, rhs, cw, nlines, tb, st); return true;
+    case 128: launch_x<T, 128>(fwd, rhs, cw, nlines, tb, st); return true;
+    case 256: launch_x<T, 256>(fwd, rhs, cw, nlines, tb, st); return true;
+    case 512: launch_x<T, 512>(fwd, rhs, cw, nlines, tb, st); return true;
+    case 1024: launch_x<T, 1024>(fwd, rhs, cw, nlines, tb, st); return true;
+    default: return false;
+  }
+}
+
 template <typename T>
 int poisson_solve(Context &c, T *rhs, hipStream_t st) {
   auto *P = static_cast<PoissonPlan *>(c.poisson);
@@ -523,42 +682,14 @@ int poisson_solve(Context &c, T *rhs, hipStream_t st) {
   PoissonTables<T> tb;
   if constexpr (sizeof(T) == 8) tb = P->t64;
   else tb = P->t32;
-  const int nlines = P->ny * P->nz;
-  // pass 1
-  {
-    const int M = P->nx / 2, TL = FFT_BLOCK_POINTS / M;
-    const size_t sh = sizeof(cplx<T>) * (size_t)M * (TL + 1);
-    hipLaunchKernelGGL((r2c_x_kernel<T>), dim3((nlines + TL - 1) / TL), dim3(FFT_THREADS), sh, st, rhs, cw, P->nx,
-                       nlines, tb);
-  }
-  // pass 2
-  {
-    const int N = P->ny, TL = FFT_BLOCK_POINTS / N;
-    const size_t sh = sizeof(cplx<T>) * (size_t)N * (TL + 1);
-    hipLaunchKernelGGL((strided_kernel<T, FFT_Y_FWD>), dim3((P->nxh + TL - 1) / TL, P->nz), dim3(FFT_THREADS), sh, st,
-                       cw, P->nxh, P->ny, P->nz, P->ydct, P->scale, 0, tb);
-  }
-  // pass 3
-  {
-    const int N = P->nz, TL = FFT_BLOCK_POINTS / N;
-    const size_t sh = sizeof(cplx<T>) * (size_t)N * (TL + 1);
-    hipLaunchKernelGGL((strided_kernel<T, FFT_Z_SOLVE>), dim3((P->nxh + TL - 1) / TL, P->ny), dim3(FFT_THREADS), sh,
-                       st, cw, P->nxh, P->ny, P->nz, P->ydct, P->scale, P->ydct, tb);
-  }
-  // pass 4
-  {
-    const int N = P->ny, TL = FFT_BLOCK_POINTS / N;
-    const size_t sh = sizeof(cplx<T>) * (size_t)N * (TL + 1);
-    hipLaunchKernelGGL((strided_kernel<T, FFT_Y_INV>), dim3((P->nxh + TL - 1) / TL, P->nz), dim3(FFT_THREADS), sh, st,
-                       cw, P->nxh, P->ny, P->nz, P->ydct, P->scale, 0, tb);
-  }
-  // pass 5
-  {
-    const int M = P->nx / 2, TL = FFT_BLOCK_POINTS / M;
-    const size_t sh = sizeof(cplx<T>) * (size_t)M * (TL + 1);
-    hipLaunchKernelGGL((c2r_x_kernel<T>), dim3((nlines + TL - 1) / TL), dim3(FFT_THREADS), sh, st, cw, rhs, P->nx,
-                       nlines, tb);
-  }
+  const int nlines = P->ny * P->nz, M = P->nx / 2;
+  // sizes were checked by poisson_setup: every dispatch below finds its case
+  bool ok = dispatch_x<T>(true, M, rhs, cw, nlines, tb, st);
+  ok = ok && dispatch_s<T, FFT_Y_FWD>(P->ny, *P, cw, tb, st);
+  ok = ok && dispatch_s<T, FFT_Z_SOLVE>(P->nz, *P, cw, tb, st);
+  ok = ok && dispatch_s<T, FFT_Y_INV>(P->ny, *P, cw, tb, st);
+  ok = ok && dispatch_x<T>(false, M, rhs, cw, nlines, tb, st);
+  if (!ok) return fail(X3D_ERR_UNSUPPORTED, "Poisson: no FFT kernel for this size");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("poisson launch: ") + hipGetErrorString(e));
   return X3D_OK;
