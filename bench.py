@@ -98,6 +98,79 @@ def _traffic(n):
     return None
 
 
+def micro_field(shape, walls, dtype, seed):
+    """Kernel-microbench input: seeded uniform[-1,1] noise plus 8 sine modes
+    of integer wavenumbers |k|_inf <= 4 (amplitudes U[0.5,1], phases
+    U[0,2pi)), on the (nz, ny, nx) grid of the [0,2pi)^3 box (y spans
+    [0,2pi] closed when y has walls)."""
+    import numpy as np
+    import torch
+    nz, ny, nx = shape
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    u = torch.rand(shape, generator=g, device="cuda", dtype=torch.float64) * 2.0 - 1.0
+    two_pi = 2.0 * math.pi
+    x = torch.arange(nx, device="cuda", dtype=torch.float64) * (two_pi / nx)
+    y = torch.arange(ny, device="cuda", dtype=torch.float64) * (two_pi / ((ny - 1) if walls else ny))
+    z = torch.arange(nz, device="cuda", dtype=torch.float64) * (two_pi / nz)
+    for _ in range(8):
+        k = rng.integers(-4, 5, size=3)
+        while not k.any():
+            k = rng.integers(-4, 5, size=3)
+        a, ph = rng.uniform(0.5, 1.0), rng.uniform(0.0, two_pi)
+        u += a * torch.sin(k[0] * x[None, None, :] + k[1] * y[None, :, None] + k[2] * z[:, None, None] + ph)
+    return u.to(dtype)
+
+
+def microbench(X, n, walls, stream, reps=10):
+    """Kernel microbench on an n^3 field (n x (n+1) x n with Dirichlet walls,
+    ncl=2, in y): first and second derivatives along x, y, z, one pressure
+    Poisson solve on the zero-mean pressure-grid field, one pencil transpose;
+    FP64 and FP32.  Times are HIP-event averages over `reps` launches on the
+    library stream.  Algorithmic bytes: 2*N*w per derivative and transpose,
+    10*N*w per Poisson solve (five FFT passes, divide fused)."""
+    import torch
+    ny = n + 1 if walls else n
+    p = X.make_params(n, ny, n, ncly1=2 if walls else 0, nclyn=2 if walls else 0, re=1600.0, dt=0.002)
+    X.init(p)
+    X.set_stream(stream.cuda_stream)
+    out = []
+    try:
+        for prec, dt in ((0, torch.float64), (1, torch.float32)):
+            w = 8 if prec == 0 else 4
+            with torch.cuda.stream(stream):
+                u = micro_field((n, ny, n), walls, dt, 20261202)
+                t = torch.empty_like(u)
+                N = u.numel()
+                for d, name in ((0, "x"), (1, "y"), (2, "z")):
+                    for op, lab in (("der1", "der"), ("der2", "der" + name)):
+                        nm = f"{lab}{name}" if op == "der1" else f"der{name}{name}"
+                        X.line_op_async(d, op, 1, u, t)
+                        ms = _events_time(stream, lambda: X.line_op_async(d, op, 1, u, t), reps)
+                        gbs = 2.0 * N * w / (ms * 1e-3) / 1e9
+                        out.append({"op": nm, "dtype": "f64" if prec == 0 else "f32", "ms": round(ms, 4),
+                                    "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+                rhs = u[:, : (ny - 1 if walls else ny), :].contiguous()
+                rhs -= rhs.mean()
+                Np = rhs.numel()
+                X.poisson_async(rhs)
+                ms = _events_time(stream, lambda: X.poisson_async(rhs), reps)
+                gbs = 10.0 * Np * w / (ms * 1e-3) / 1e9
+                out.append({"op": "poisson_010" if walls else "poisson_000", "dtype": "f64" if prec == 0 else "f32",
+                            "ms": round(ms, 4), "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+                X.transpose_async(u, t)
+                ms = _events_time(stream, lambda: X.transpose_async(u, t), reps)
+                gbs = 2.0 * N * w / (ms * 1e-3) / 1e9
+                out.append({"op": "transpose", "dtype": "f64" if prec == 0 else "f32", "ms": round(ms, 4),
+                            "GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
+                del u, t, rhs
+                torch.cuda.synchronize()
+    finally:
+        X.finalize()
+    return {"grid": [n, ny, n], "y": "Dirichlet walls (ncl=2)" if walls else "periodic", "ops": out}
+
+
 def cpu_baseline(n, steps=1):
     """The oracle (numpy, single thread) on a bounded sample of the same
     workload: `steps` TGV time steps at n^3 after one untimed step."""
@@ -126,6 +199,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--micro-n", type=int, default=512,
+                    help="kernel microbench grid (0 = skip): n^3 periodic and n x (n+1) x n with y walls")
     ap.add_argument("--only", choices=["fp64", "fp32", "both"], default="both")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the roofline kernel (and a copy of known size) -- for counter runs")
@@ -195,10 +270,12 @@ def main():
             f32["ke_rel_err_vs_fp64"] = abs(r32["stats"][0] - r64["stats"][0]) / abs(r64["stats"][0])
         out["fp32_bulk"] = f32
     out["device"] = X.device_name()
+    X.finalize()
+    if args.micro_n > 0:
+        out["microbench"] = [microbench(X, args.micro_n, False, stream), microbench(X, args.micro_n, True, stream)]
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
         out["cpu_baseline"]["host"] = f"{os.cpu_count()} logical CPUs visible"
-    X.finalize()
     print(json.dumps(out))
 
 

@@ -198,6 +198,12 @@ int x3d_line_op_async(int dir, int op, int npaire, int precision, const void *u,
  * wavenumber in FP64. */
 int x3d_poisson(double *rhs, const int *n1, const int *n2, const int *n3);
 int x3d_poisson_f32(float *rhs, const int *n1, const int *n2, const int *n3);
+/* Enqueue-only variants for whole-step callers (no counterpart in the
+ * reference, whose calls are synchronous): the work is queued on the library
+ * stream and the call returns at once.  precision 0 = FP64, 1 = FP32.
+ * x3d_transpose_async stands for all four transposes (one GPU: a copy). */
+int x3d_poisson_async(int precision, void *rhs, int n1, int n2, int n3);
+int x3d_transpose_async(int precision, const void *src, void *dst, int n1, int n2, int n3);
 
 /* ---- pencil transposes ----------------------------------------------------
  * 2DECOMP&FFT v2.0.4 transpose_x_to_y / y_to_z / z_to_y / y_to_x (external;
@@ -208,9 +214,14 @@ int x3d_transpose_x_to_y(const double *src, double *dst, const int *n1, const in
 int x3d_transpose_y_to_z(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
 int x3d_transpose_z_to_y(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
 int x3d_transpose_y_to_x(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
-/* Explicit pencil reorder (i,j,k) -> permuted storage, LDS-tiled: perm is
- * 0 = (j,i,k) [y-fastest], 1 = (k,j,i)... see x3d_reorder() documentation in
- * the package; used when a pencil is stored with its own axis fastest. */
+/* Single-precision builds of the reference (mytype = real(4)). */
+int x3d_transpose_x_to_y_f32(const float *src, float *dst, const int *n1, const int *n2, const int *n3);
+int x3d_transpose_y_to_z_f32(const float *src, float *dst, const int *n1, const int *n2, const int *n3);
+int x3d_transpose_z_to_y_f32(const float *src, float *dst, const int *n1, const int *n2, const int *n3);
+int x3d_transpose_y_to_x_f32(const float *src, float *dst, const int *n1, const int *n2, const int *n3);
+/* LDS-tiled layout change dst(j,i,k) = src(i,j,k) of an (n1,n2,n3) FP64
+ * array, out of place: the reorder a y-pencil needs when it is stored with
+ * y fastest.  No counterpart in the reference (its pencils keep i fastest). */
 int x3d_reorder_ji(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
 
 /* ---- host-memory entries (arrays in host memory; staged through HBM) ---- */
@@ -225,10 +236,11 @@ int x3d_host_poisson(double *rhs, const int *n1, const int *n2, const int *n3);
  * precision: 0 = FP64 everywhere; 1 = FP32 bulk (FP32 derivatives, FFTs,
  * convective products) with FP64 velocity accumulation, FP64 Poisson divide
  * and FP64 diagnostics reductions.
- * For nx in {64,128,256,512} and ny, nz <= 512 the momentum right-hand side
- * and AB3 run as nine fused sweeps; other grids (or the environment variable
- * X3D_UNFUSED_MOMENTUM set at create time) use one launch per derivative.
- * Both paths compute the same quantities. */
+ * For nx in {64,128,256,512} and ny, nz multiples of 16 up to 512 the
+ * momentum right-hand side and AB3 run as nine fused sweeps and the
+ * divergence / gradient + cor_vel as six; other grids (or the environment
+ * variable X3D_UNFUSED_MOMENTUM set at create time) use one launch per
+ * derivative.  Both paths compute the same quantities. */
 typedef struct x3d_solver x3d_solver;
 int x3d_solver_create(int precision, x3d_solver **out);
 int x3d_solver_destroy(x3d_solver *s);

@@ -109,11 +109,29 @@ def poisson(rhs) -> None:
     check(fn(derive._dptr(rhs), iref(n1), iref(n2), iref(n3)), "poisson")
 
 
+def poisson_async(rhs) -> None:
+    """x3d_poisson_async: poisson(rhs) queued on the library stream, no wait."""
+    import torch
+    n1, n2, n3 = _dims(rhs)
+    check(_lib.lib().x3d_poisson_async(0 if rhs.dtype == torch.float64 else 1, derive._dptr(rhs), n1, n2, n3),
+          "poisson_async")
+
+
+def transpose_async(src, dst) -> None:
+    """x3d_transpose_async: any of the four pencil transposes, queued only."""
+    import torch
+    n1, n2, n3 = _dims(src)
+    check(_lib.lib().x3d_transpose_async(0 if src.dtype == torch.float64 else 1, derive._dptr(src),
+                                          derive._dptr(dst), n1, n2, n3), "transpose_async")
+
+
 def _transpose(name):
     def f(src, dst):
+        import torch
         n1, n2, n3 = _dims(src)
-        check(getattr(_lib.lib(), f"x3d_{name}")(derive._dptr(src), derive._dptr(dst), iref(n1),
-                                                  iref(n2), iref(n3)), name)
+        sfx = "" if src.dtype == torch.float64 or name == "reorder_ji" else "_f32"
+        check(getattr(_lib.lib(), f"x3d_{name}{sfx}")(derive._dptr(src), derive._dptr(dst), iref(n1),
+                                                       iref(n2), iref(n3)), name)
     f.__name__ = name
     return f
 

@@ -74,8 +74,12 @@ def _declare(lib):
     for nm in ("x3d_transpose_x_to_y", "x3d_transpose_y_to_z", "x3d_transpose_z_to_y",
                "x3d_transpose_y_to_x", "x3d_reorder_ji"):
         getattr(lib, nm).argtypes = [P, P, IP, IP, IP]
+        if nm != "x3d_reorder_ji":
+            getattr(lib, nm + "_f32").argtypes = [P, P, IP, IP, IP]
     lib.x3d_line_op_async.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P, P, C.c_int, C.c_int,
                                       C.c_int]
+    lib.x3d_poisson_async.argtypes = [C.c_int, P, C.c_int, C.c_int, C.c_int]
+    lib.x3d_transpose_async.argtypes = [C.c_int, P, P, C.c_int, C.c_int, C.c_int]
     lib.x3d_host_derx_00.argtypes = [P, P, P, P, P, P, P, IP, IP, IP, IP, P]
     lib.x3d_host_poisson.argtypes = [P, IP, IP, IP]
     lib.x3d_solver_create.argtypes = [C.c_int, C.POINTER(P)]

@@ -469,15 +469,21 @@ int x3d_line_op_async(int dir, int op, int npaire, int precision, const void *u,
 
 // ---------------------------------------------------------------- Poisson
 template <typename T>
+static int poisson_enqueue(Context *c, T *rhs, int n1, int n2, int n3) {
+  if (n1 <= 0 || n2 <= 0 || n3 <= 0) return fail(X3D_ERR_ARG, "non-positive size");
+  if (!rhs) return fail(X3D_ERR_ARG, "null rhs");
+  if (!c->poisson) return fail(X3D_ERR_UNSUPPORTED, "no Poisson solver for these boundary conditions/sizes");
+  if (n1 != c->nxm || n2 != c->nym || n3 != c->nzm)
+    return fail(X3D_ERR_ARG, "rhs extents differ from the pressure grid (nxm, nym, nzm)");
+  return poisson_solve<T>(*c, rhs, c->stream);
+}
+
+template <typename T>
 static int poisson_entry(T *rhs, const int *n1, const int *n2, const int *n3) {
   Context *c;
   X3D_TRY(require_ctx(&c));
   X3D_TRY(check_dims(n1, n2, n3));
-  if (!rhs) return fail(X3D_ERR_ARG, "null rhs");
-  if (!c->poisson) return fail(X3D_ERR_UNSUPPORTED, "no Poisson solver for these boundary conditions/sizes");
-  if (*n1 != c->nxm || *n2 != c->nym || *n3 != c->nzm)
-    return fail(X3D_ERR_ARG, "rhs extents differ from the pressure grid (nxm, nym, nzm)");
-  X3D_TRY(poisson_solve<T>(*c, rhs, c->stream));
+  X3D_TRY(poisson_enqueue<T>(c, rhs, *n1, *n2, *n3));
   X3D_HIP(hipStreamSynchronize(c->stream));
   return X3D_OK;
 }
@@ -490,29 +496,59 @@ int x3d_poisson(double *rhs, const int *n1, const int *n2, const int *n3) {
 int x3d_poisson_f32(float *rhs, const int *n1, const int *n2, const int *n3) {
   return poisson_entry<float>(rhs, n1, n2, n3);
 }
+int x3d_poisson_async(int precision, void *rhs, int n1, int n2, int n3) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (precision == 0) return poisson_enqueue<double>(c, static_cast<double *>(rhs), n1, n2, n3);
+  if (precision == 1) return poisson_enqueue<float>(c, static_cast<float *>(rhs), n1, n2, n3);
+  return fail(X3D_ERR_ARG, "precision must be 0 or 1");
+}
+
+}  // extern "C"
 
 // ---------------------------------------------------------------- transposes
-static int transpose_entry(const double *src, double *dst, const int *n1, const int *n2, const int *n3) {
+template <typename T>
+static int transpose_enqueue(Context *c, const T *src, T *dst, int n1, int n2, int n3) {
+  if (n1 <= 0 || n2 <= 0 || n3 <= 0) return fail(X3D_ERR_ARG, "non-positive size");
+  if (!src || !dst) return fail(X3D_ERR_ARG, "null pointer");
+  const size_t n = (size_t)n1 * n2 * n3;
+  if (src != dst) X3D_TRY(launch_copy<T>(src, dst, n, c->stream));
+  return X3D_OK;
+}
+
+template <typename T>
+static int transpose_entry(const T *src, T *dst, const int *n1, const int *n2, const int *n3) {
   Context *c;
   X3D_TRY(require_ctx(&c));
   X3D_TRY(check_dims(n1, n2, n3));
-  if (!src || !dst) return fail(X3D_ERR_ARG, "null pointer");
-  const size_t n = (size_t)*n1 * *n2 * *n3;
-  if (src != dst) X3D_TRY(launch_copy<double>(src, dst, n, c->stream));
+  X3D_TRY(transpose_enqueue<T>(c, src, dst, *n1, *n2, *n3));
   X3D_HIP(hipStreamSynchronize(c->stream));
   return X3D_OK;
 }
-int x3d_transpose_x_to_y(const double *s, double *d, const int *a, const int *b, const int *c) {
-  return transpose_entry(s, d, a, b, c);
-}
-int x3d_transpose_y_to_z(const double *s, double *d, const int *a, const int *b, const int *c) {
-  return transpose_entry(s, d, a, b, c);
-}
-int x3d_transpose_z_to_y(const double *s, double *d, const int *a, const int *b, const int *c) {
-  return transpose_entry(s, d, a, b, c);
-}
-int x3d_transpose_y_to_x(const double *s, double *d, const int *a, const int *b, const int *c) {
-  return transpose_entry(s, d, a, b, c);
+
+extern "C" {
+
+#define X3D_DEF_TRANSPOSE(name, T)                                                            \
+  int name(const T *s, T *d, const int *a, const int *b, const int *c) {                     \
+    return transpose_entry<T>(s, d, a, b, c);                                                 \
+  }
+X3D_DEF_TRANSPOSE(x3d_transpose_x_to_y, double)
+X3D_DEF_TRANSPOSE(x3d_transpose_y_to_z, double)
+X3D_DEF_TRANSPOSE(x3d_transpose_z_to_y, double)
+X3D_DEF_TRANSPOSE(x3d_transpose_y_to_x, double)
+X3D_DEF_TRANSPOSE(x3d_transpose_x_to_y_f32, float)
+X3D_DEF_TRANSPOSE(x3d_transpose_y_to_z_f32, float)
+X3D_DEF_TRANSPOSE(x3d_transpose_z_to_y_f32, float)
+X3D_DEF_TRANSPOSE(x3d_transpose_y_to_x_f32, float)
+
+int x3d_transpose_async(int precision, const void *src, void *dst, int n1, int n2, int n3) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (precision == 0)
+    return transpose_enqueue<double>(c, static_cast<const double *>(src), static_cast<double *>(dst), n1, n2, n3);
+  if (precision == 1)
+    return transpose_enqueue<float>(c, static_cast<const float *>(src), static_cast<float *>(dst), n1, n2, n3);
+  return fail(X3D_ERR_ARG, "precision must be 0 or 1");
 }
 int x3d_reorder_ji(const double *src, double *dst, const int *n1, const int *n2, const int *n3) {
   Context *c;

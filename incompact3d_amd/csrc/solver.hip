@@ -3,10 +3,12 @@
 //   int_time (AB3)         time_integrators.f90:85-100
 //   solve_poisson          navier.f90:27-141    (divergence, poisson, gradp)
 //   cor_vel                navier.f90:206-249
-// as in the time loop of xcompact3d.f90:29-102.  Every derivative and
-// staggered operator is one line-solve launch whose epilogue does the
-// reference's array arithmetic (products, sums, the pressure correction), so
-// no separate elementwise pass touches HBM.  precision 0 computes in FP64;
+// as in the time loop of xcompact3d.f90:29-102.  On grids the fused sweeps
+// cover, each direction of the momentum right-hand side is one launch per
+// velocity component (AB3 fused into the x sweep) and each direction of the
+// divergence and of the gradient + cor_vel is one launch; elsewhere every
+// derivative and staggered operator is one line-solve launch whose epilogue
+// does the reference's array arithmetic.  precision 0 computes in FP64;
 // precision 1 keeps an FP32 copy of the velocity for every derivative, FFT
 // and product, while the velocity itself is accumulated in FP64 (AB3 and
 // cor_vel) and the Poisson divide and reductions run in FP64.
@@ -32,18 +34,13 @@ int launch_tgv_sums(const T *ux, const T *uy, const T *uz, const T *const g[9], 
                     const T *lz, double xnu, size_t n, double *partial, double *out4, hipStream_t st);
 template <typename T>
 int launch_max_mean(const T *a, size_t n, double *partial, double *out2, hipStream_t st);
-bool momentum_fusable(const Context &c);
-template <typename T>
-int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], const T *acc, T *out,
-                    const T *f2, const T *f3, const double *u_old, double *u_new, float *shadow, double ca,
-                    double cb, double cc, int nterms, int nx, int ny, int nz, hipStream_t st);
 
 }  // namespace x3d
 
 struct x3d_solver {
   int precision = 0;
   int itime = 0;
-  bool fused = false;          // fused momentum sweeps (momentum.hip) cover this grid
+  bool fused = false;          // the fused sweeps (momentum.hip, projection.hip) cover this grid
   size_t n = 0;
   int nx = 0, ny = 0, nz = 0;
   double *u64[3] = {nullptr, nullptr, nullptr};   // current velocity (FP64)
@@ -205,9 +202,34 @@ int step_once(Context &c, x3d_solver *S) {
   S->slot[1] = s1;
   S->slot[2] = s2;
   // projection
-  X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));
-  X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
-  X3D_TRY(gradp_correct<T>(c, S, u, W));
+  if (S->fused) {
+    T *const onone[3] = {nullptr, nullptr, nullptr};
+    const T *const dx_in[3] = {cu[0], cu[1], cu[2]};
+    T *const dx_out[3] = {W[0], W[1], W[2]};
+    X3D_TRY(launch_projection<T>(c, SW_DIV_X, dx_in, dx_out, nullptr, nullptr, nx, ny, nz, c.stream));
+    const T *const dy_in[3] = {W[0], W[1], W[2]};
+    T *const dy_out[3] = {W[3], W[1], nullptr};
+    X3D_TRY(launch_projection<T>(c, SW_DIV_Y, dy_in, dy_out, nullptr, nullptr, nx, ny, nz, c.stream));
+    const T *const dz_in[3] = {W[3], W[1], nullptr};
+    T *const dz_out[3] = {W[0], nullptr, nullptr};
+    X3D_TRY(launch_projection<T>(c, SW_DIV_Z, dz_in, dz_out, nullptr, nullptr, nx, ny, nz, c.stream));
+    X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
+    const T *const gz_in[3] = {W[0], nullptr, nullptr};
+    T *const gz_out[3] = {W[1], W[2], nullptr};
+    X3D_TRY(launch_projection<T>(c, SW_GRAD_Z, gz_in, gz_out, nullptr, nullptr, nx, ny, nz, c.stream));
+    const T *const gy_in[3] = {W[1], W[2], nullptr};
+    T *const gy_out[3] = {W[3], W[0], W[1]};
+    X3D_TRY(launch_projection<T>(c, SW_GRAD_Y, gy_in, gy_out, nullptr, nullptr, nx, ny, nz, c.stream));
+    const T *const gx_in[3] = {W[3], W[0], W[1]};
+    double *const u64[3] = {S->u64[0], S->u64[1], S->u64[2]};
+    float *const u32[3] = {S->u32[0], S->u32[1], S->u32[2]};
+    X3D_TRY(launch_projection<T>(c, SW_GRAD_X, gx_in, onone, u64, sizeof(T) == 8 ? nullptr : u32, nx, ny, nz,
+                                 c.stream));
+  } else {
+    X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));
+    X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
+    X3D_TRY(gradp_correct<T>(c, S, u, W));
+  }
   return X3D_OK;
 }
 
@@ -327,7 +349,7 @@ int x3d_solver_create(int precision, x3d_solver **out) {
   S->ny = c->p.ny;
   S->nz = c->p.nz;
   S->n = (size_t)S->nx * S->ny * S->nz;
-  S->fused = momentum_fusable(*c) && getenv("X3D_UNFUSED_MOMENTUM") == nullptr;
+  S->fused = momentum_fusable(*c) && projection_fusable(*c) && getenv("X3D_UNFUSED_MOMENTUM") == nullptr;
   int r = alloc_solver(*c, S);
   if (r != X3D_OK) {
     free_solver(S);

@@ -153,4 +153,20 @@ void poisson_free(Context &c);
 template <typename T>
 int poisson_solve(Context &c, T *rhs, hipStream_t st);
 
+// ---------------------------------------------------------------- fused sweeps
+// momentum.hip: the transeq right-hand side of component comp along dir
+// (out = acc + ...); the x sweep also applies AB3 (nterms > 0) into u_new.
+bool momentum_fusable(const Context &c);
+template <typename T>
+int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], const T *acc, T *out,
+                    const T *f2, const T *f3, const double *u_old, double *u_new, float *shadow, double ca,
+                    double cb, double cc, int nterms, int nx, int ny, int nz, hipStream_t st);
+
+// projection.hip: divergence (x, y, z) and gradient + cor_vel (z, y, x)
+enum { SW_DIV_X = 0, SW_DIV_Y, SW_DIV_Z, SW_GRAD_Z, SW_GRAD_Y, SW_GRAD_X };
+bool projection_fusable(const Context &c);
+template <typename T>
+int launch_projection(const Context &c, int sw, const T *const in[3], T *const out[3], double *const u64[3],
+                      float *const u32[3], int nx, int ny, int nz, hipStream_t st);
+
 }  // namespace x3d

@@ -1,8 +1,10 @@
-"""GPU parity of the fused momentum sweeps (one launch per sweep direction and
-velocity component, AB3 fused into the x sweep) against the oracle's time
-step (transeq.f90:73-470 + time_integrators.f90:85-100 + navier.f90), and
-against the unfused path (one line-solve launch per derivative, selected with
-X3D_UNFUSED_MOMENTUM at solver creation).
+"""GPU parity of the fused time step -- momentum sweeps (one launch per sweep
+direction and velocity component, AB3 fused into the x sweep) and projection
+sweeps (one launch per direction for the divergence and for gradp + cor_vel)
+-- against the oracle's time step (transeq.f90:73-470 +
+time_integrators.f90:85-100 + navier.f90), and against the unfused path (one
+line-solve launch per derivative, selected with X3D_UNFUSED_MOMENTUM at
+solver creation).
 
 Grids: the fused sweeps need nx in {64,128,256,512} and ny, nz <= 512; the
 non-cubic cases put a different segment count on each sweep.
