@@ -64,6 +64,32 @@ def run_solver(X, p, precision, warmup, steps, stream):
             "stats": stats, "div": div}
 
 
+def run_les_step(X, p, warmup, steps, stream):
+    """FP32-bulk step with the learned closure (eddy-viscosity form, model 3)
+    at the bench grid: nine gradients, the BF16-MFMA network, its stress
+    divergence, then the step.  The network is freshly initialised -- the
+    cost does not depend on the weights; its trained accuracy is measured by
+    tests/test_gpu_les_learned.py."""
+    import torch
+    from incompact3d_amd import les
+    s = X.Solver(p, 1)
+    s.init_tgv()
+    m = X.Mlp(les.init_params(0))
+    delta = (p.xlx * p.yly * p.zlz / (p.nx * p.ny * p.nz)) ** (1.0 / 3.0)
+    s.set_les(3, cs=0.17, beta=1.0, mlp=m, gscale=1.0, delta0=delta)
+    s.step(warmup)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    s.step(steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    s.close()
+    m.close()
+    return ev0.elapsed_time(ev1) / steps
+
+
 def roofline_line_kernel(X, n, stream, reps=40):
     """Average duration of the dominant kernel -- the FP64 line-solve kernel
     of a first derivative along y on the n^3 field -- measured with HIP events
@@ -234,6 +260,7 @@ def main():
         r64 = run_solver(X, p, 0, args.warmup, args.steps, stream) if args.only != "fp32" else None
         r32 = run_solver(X, p, 1, args.warmup, args.steps, stream) if args.only != "fp64" else None
         roof = roofline_line_kernel(X, n, stream)
+        les_ms = run_les_step(X, p, args.warmup, args.steps, stream) if args.only == "both" else None
     npts = n ** 3
     main_r = r64 if r64 is not None else r32
     ms = main_r["event_ms_per_step"]
@@ -269,6 +296,11 @@ def main():
             f32["eps_rel_err_vs_fp64"] = abs(e32 - e64) / abs(e64)
             f32["ke_rel_err_vs_fp64"] = abs(r32["stats"][0] - r64["stats"][0]) / abs(r64["stats"][0])
         out["fp32_bulk"] = f32
+    if les_ms is not None:
+        out["fp32_learned_les"] = {"value": round(npts / (les_ms * 1e-3) / 1e6, 2), "ms_per_step": round(les_ms, 4),
+                                   "model": "closure network 10-64-64-64-6 on BF16 MFMA, eddy-viscosity form, "
+                                            "Smagorinsky fallback (Cs 0.17, beta 1)",
+                                   "speedup_vs_fp64": round((npts / (les_ms * 1e-3) / 1e6) / value, 3)}
     out["device"] = X.device_name()
     X.finalize()
     if args.micro_n > 0:

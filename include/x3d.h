@@ -260,6 +260,58 @@ int x3d_solver_tgv_stats(x3d_solver *s, double out4[4]);
 int x3d_solver_divergence_stats(x3d_solver *s, double out2[2]);
 int x3d_solver_itime(x3d_solver *s, int *itime);
 
+/* ---- LES subgrid stress (les_models.f90) ----------------------------------
+ * model 0: none (DNS); 1: Smagorinsky, jles=1 of compute_SGS
+ * (les_models.f90:96-165) with smagcst = cs and iconserv 0
+ * (sgs_mom_nonconservative, :1091-1250) or 1 (sgs_mom_conservative,
+ * :1327-1456); 2: the learned closure network `mlp` (no counterpart in the
+ * reference) -- T_ij from the nine resolved gradients over gscale and the
+ * filter width over delta0, divergence with the conservative parities, and
+ * the Smagorinsky stress of constant cs wherever T_ij S_ij < -beta 2 nut S:S;
+ * 3: the same network, keeping only the dissipative part of its stress
+ * along the strain, T = 2 nu_t S with nu_t = max(T:S, 0) / (2 S:S).
+ * The force is added to the momentum right-hand side before AB3, as
+ * transeq.f90:498-503 adds sgsx1..sgsz1.  The solver does not own mlp. */
+typedef struct x3d_mlp x3d_mlp;
+int x3d_solver_set_les(x3d_solver *s, int model, double cs, int iconserv, double beta, x3d_mlp *mlp,
+                       double gscale, double delta0);
+/* SGS force of the current velocity into device FP64 arrays (nx,ny,nz). */
+int x3d_solver_sgs(x3d_solver *s, double *sgsx, double *sgsy, double *sgsz);
+/* Points where the learned closure fell back to Smagorinsky since the last
+ * call (the counter is reset). */
+int x3d_solver_les_fallbacks(x3d_solver *s, long long *count);
+
+/* ---- closure network (BF16 MFMA, FP32 accumulation) ------------------------
+ * 10 -> 64 -> 64 -> 64 -> 6 with GELU; x3d_mlp_nparams() FP32 parameters in
+ * the order W1 (64x10, row-major), b1, W2 (64x64), b2, W3, b3, W4 (6x64), b4.
+ * Feature rows (n x 10, device FP32): g_xx g_yx g_zx g_xy g_yy g_zy g_xz g_yz
+ * g_zz (g_ij = d u_i / d x_j) over gscale, then delta/delta0.  Output rows
+ * (n x 6): T_xx T_yy T_zz T_xy T_xz T_yz over delta^2 gscale^2.
+ * Training: mean squared error over the n x 6 entries, Adam (lr, beta1 0.9,
+ * beta2 0.999, eps 1e-8) on FP32 master weights.  Not re-entrant. */
+int x3d_mlp_nparams(void);
+int x3d_mlp_create(x3d_mlp **out);
+int x3d_mlp_destroy(x3d_mlp *m);
+int x3d_mlp_set_params(x3d_mlp *m, const float *host_params);   /* also resets Adam */
+int x3d_mlp_get_params(x3d_mlp *m, float *host_params);
+int x3d_mlp_forward(x3d_mlp *m, const float *x, float *y, int n);
+/* One Adam step on a batch; *loss (host, may be NULL) = the batch MSE. */
+int x3d_mlp_train_step(x3d_mlp *m, const float *x, const float *t, int n, float lr, float *loss);
+/* Gradient of the batch MSE into host_grad (x3d_mlp_nparams() floats), no update. */
+int x3d_mlp_gradient(x3d_mlp *m, const float *x, const float *t, int n, float *host_grad, float *loss);
+/* T_ij (device, precision 0 FP64 / 1 FP32) from the nine gradient fields g
+ * (order as the features), with the Smagorinsky fallback; *nfallback (host,
+ * may be NULL) counts the fallback points. */
+int x3d_mlp_closure(x3d_mlp *m, int precision, const void *const *g, void *const *t, long long n, double gscale,
+                    double delta, double delta0, double cs, double beta, long long *nfallback);
+
+/* Top-hat filter of a device FP64 velocity (nx,ny,nz) over f^3 cells onto
+ * (nx/f, ny/f, nz/f): ub[3] the filtered velocity, tau[6] (xx yy zz xy xz yz)
+ * = bar(u_i u_j) - bar(u_i) bar(u_j).  Training data for the closure; no
+ * counterpart in the reference. */
+int x3d_box_filter(const double *ux, const double *uy, const double *uz, int nx, int ny, int nz, int f,
+                   double *const *ub, double *const *tau);
+
 #ifdef __cplusplus
 }
 #endif

@@ -201,3 +201,113 @@ class Solver:
         check(_lib.lib().x3d_solver_itime(self._h, C.byref(v)), "itime")
         return v.value
 
+    def set_les(self, model: int, cs: float = 0.15, iconserv: int = 0, beta: float = 0.0, mlp=None,
+                gscale: float = 1.0, delta0: float = 1.0):
+        """x3d_solver_set_les: 0 none, 1 Smagorinsky (jles=1, smagcst=cs,
+        iconserv), 2 the learned closure ``mlp`` (an Mlp) with the Smagorinsky
+        fallback of constant cs and bound beta."""
+        self._mlp = mlp   # keep the network alive while the solver uses it
+        check(_lib.lib().x3d_solver_set_les(self._h, int(model), float(cs), int(iconserv), float(beta),
+                                            mlp._h if mlp is not None else None, float(gscale),
+                                            float(delta0)), "set_les")
+
+    def sgs(self):
+        """SGS force (sgsx, sgsy, sgsz) of the current velocity, FP64 device tensors."""
+        import torch
+        out = [torch.empty(self._shape, dtype=torch.float64, device="cuda") for _ in range(3)]
+        check(_lib.lib().x3d_solver_sgs(self._h, *[derive._dptr(t) for t in out]), "sgs")
+        return out
+
+    def les_fallbacks(self):
+        v = C.c_longlong(0)
+        check(_lib.lib().x3d_solver_les_fallbacks(self._h, C.byref(v)), "les_fallbacks")
+        return v.value
+
+
+def _ptr_array(ts):
+    arr = (C.c_void_p * len(ts))()
+    for i, t in enumerate(ts):
+        arr[i] = derive._dptr(t)
+    return arr
+
+
+class Mlp:
+    """The closure network (x3d_mlp_*): 10-64-64-64-6, GELU, BF16 MFMA."""
+
+    def __init__(self, params=None):
+        self._h = C.c_void_p()
+        check(_lib.lib().x3d_mlp_create(C.byref(self._h)), "x3d_mlp_create")
+        self.nparams = _lib.lib().x3d_mlp_nparams()
+        if params is not None:
+            self.set_params(params)
+
+    def close(self):
+        if self._h:
+            _lib.lib().x3d_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, flat):
+        import numpy as np
+        a = np.ascontiguousarray(np.asarray(flat, dtype=np.float32).ravel())
+        if a.size != self.nparams:
+            raise ValueError(f"expected {self.nparams} parameters, got {a.size}")
+        check(_lib.lib().x3d_mlp_set_params(self._h, a.ctypes.data), "mlp_set_params")
+
+    def get_params(self):
+        import numpy as np
+        a = np.empty(self.nparams, dtype=np.float32)
+        check(_lib.lib().x3d_mlp_get_params(self._h, a.ctypes.data), "mlp_get_params")
+        return a
+
+    def forward(self, x):
+        """x: (n, 10) FP32 device tensor -> (n, 6)."""
+        import torch
+        y = torch.empty((x.shape[0], 6), dtype=torch.float32, device=x.device)
+        check(_lib.lib().x3d_mlp_forward(self._h, derive._dptr(x), derive._dptr(y), int(x.shape[0])),
+              "mlp_forward")
+        return y
+
+    def train_step(self, x, t, lr: float):
+        loss = C.c_float(0.0)
+        check(_lib.lib().x3d_mlp_train_step(self._h, derive._dptr(x), derive._dptr(t), int(x.shape[0]),
+                                            float(lr), C.byref(loss)), "mlp_train_step")
+        return loss.value
+
+    def gradient(self, x, t):
+        import numpy as np
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = C.c_float(0.0)
+        check(_lib.lib().x3d_mlp_gradient(self._h, derive._dptr(x), derive._dptr(t), int(x.shape[0]),
+                                          g.ctypes.data, C.byref(loss)), "mlp_gradient")
+        return g, loss.value
+
+    def closure(self, g, gscale, delta, delta0, cs, beta):
+        """T_ij (6 tensors) from the nine gradient tensors g (FP64 or FP32)."""
+        import torch
+        t = [torch.empty_like(g[0]) for _ in range(6)]
+        nfb = C.c_longlong(0)
+        prec = 0 if g[0].dtype == torch.float64 else 1
+        check(_lib.lib().x3d_mlp_closure(self._h, prec, _ptr_array(g), _ptr_array(t), int(g[0].numel()),
+                                         float(gscale), float(delta), float(delta0), float(cs), float(beta),
+                                         C.byref(nfb)), "mlp_closure")
+        return t, nfb.value
+
+
+def box_filter(ux, uy, uz, f: int):
+    """x3d_box_filter: filtered velocity (3) and tau_ij (6: xx yy zz xy xz yz)
+    on the (nz/f, ny/f, nx/f) grid; FP64 device tensors in and out."""
+    import torch
+    nz, ny, nx = ux.shape
+    shape = (nz // f, ny // f, nx // f)
+    ub = [torch.empty(shape, dtype=torch.float64, device=ux.device) for _ in range(3)]
+    tau = [torch.empty(shape, dtype=torch.float64, device=ux.device) for _ in range(6)]
+    check(_lib.lib().x3d_box_filter(derive._dptr(ux), derive._dptr(uy), derive._dptr(uz), nx, ny, nz, int(f),
+                                    _ptr_array(ub), _ptr_array(tau)), "box_filter")
+    return ub, tau
+

@@ -91,6 +91,21 @@ def _declare(lib):
     lib.x3d_solver_tgv_stats.argtypes = [P, C.POINTER(C.c_double)]
     lib.x3d_solver_divergence_stats.argtypes = [P, C.POINTER(C.c_double)]
     lib.x3d_solver_itime.argtypes = [P, IP]
+    LLP = C.POINTER(C.c_longlong)
+    lib.x3d_solver_set_les.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, P, C.c_double, C.c_double]
+    lib.x3d_solver_sgs.argtypes = [P, P, P, P]
+    lib.x3d_solver_les_fallbacks.argtypes = [P, LLP]
+    lib.x3d_mlp_nparams.argtypes = []
+    lib.x3d_mlp_create.argtypes = [C.POINTER(P)]
+    lib.x3d_mlp_destroy.argtypes = [P]
+    lib.x3d_mlp_set_params.argtypes = [P, P]
+    lib.x3d_mlp_get_params.argtypes = [P, P]
+    lib.x3d_mlp_forward.argtypes = [P, P, P, C.c_int]
+    lib.x3d_mlp_train_step.argtypes = [P, P, P, C.c_int, C.c_float, C.POINTER(C.c_float)]
+    lib.x3d_mlp_gradient.argtypes = [P, P, P, C.c_int, P, C.POINTER(C.c_float)]
+    lib.x3d_mlp_closure.argtypes = [P, C.c_int, C.POINTER(P), C.POINTER(P), C.c_longlong, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, LLP]
+    lib.x3d_box_filter.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(P), C.POINTER(P)]
 
 
 def lib():

@@ -22,10 +22,40 @@ __global__ void __launch_bounds__(EW_THREADS) copy_kernel(const T *__restrict__ 
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// 16-byte chunks, four per thread with every load issued before the first
+// store, one pass over the data (no grid-stride loop)
+constexpr int COPY_CHUNKS = 4;
+__global__ void __launch_bounds__(EW_THREADS) copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                            size_t n16) {
+  const size_t base = (size_t)blockIdx.x * (EW_THREADS * COPY_CHUNKS) + threadIdx.x;
+  uint4 v[COPY_CHUNKS];
+#pragma unroll
+  for (int k = 0; k < COPY_CHUNKS; ++k) {
+    const size_t i = base + (size_t)k * EW_THREADS;
+    if (i < n16) v[k] = src[i];
+  }
+#pragma unroll
+  for (int k = 0; k < COPY_CHUNKS; ++k) {
+    const size_t i = base + (size_t)k * EW_THREADS;
+    if (i < n16) dst[i] = v[k];
+  }
+}
+
 template <typename T>
 int launch_copy(const T *src, T *dst, size_t n, hipStream_t st) {
   if (n == 0) return X3D_OK;
-  hipLaunchKernelGGL((copy_kernel<T>), dim3(ew_blocks(n)), dim3(EW_THREADS), 0, st, src, dst, n);
+  const size_t bytes = n * sizeof(T);
+  if ((bytes % 16) == 0 && (reinterpret_cast<uintptr_t>(src) % 16) == 0 &&
+      (reinterpret_cast<uintptr_t>(dst) % 16) == 0) {
+    const size_t n16 = bytes / 16;
+    const size_t per_block = (size_t)EW_THREADS * COPY_CHUNKS;
+    const size_t blocks = (n16 + per_block - 1) / per_block;
+    if (blocks > 0x7fffffffu) return fail(X3D_ERR_UNSUPPORTED, "copy: field too large");
+    hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(EW_THREADS), 0, st,
+                       reinterpret_cast<const uint4 *>(src), reinterpret_cast<uint4 *>(dst), n16);
+  } else {
+    hipLaunchKernelGGL((copy_kernel<T>), dim3(ew_blocks(n)), dim3(EW_THREADS), 0, st, src, dst, n);
+  }
   X3D_HIP(hipGetLastError());
   return X3D_OK;
 }
@@ -122,6 +152,16 @@ __global__ void __launch_bounds__(EW_THREADS) cast_kernel(const double *__restri
 }
 int launch_cast(const double *a, float *b, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(cast_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, st, a, b, n);
+  X3D_HIP(hipGetLastError());
+  return X3D_OK;
+}
+__global__ void __launch_bounds__(EW_THREADS) widen_kernel(const float *__restrict__ a, double *__restrict__ b,
+                                                           size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = (double)a[i];
+}
+int launch_widen(const float *a, double *b, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(widen_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, st, a, b, n);
   X3D_HIP(hipGetLastError());
   return X3D_OK;
 }

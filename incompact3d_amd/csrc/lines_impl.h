@@ -209,6 +209,227 @@ __device__ T boundary_row(int i, int n_in, int n_out, const Sten<T> &s, int bc1,
   }
 }
 
+// ---------------------------------------------------------------- y / z segment solve
+// Block: TL consecutive i (one row of TL words per line point) x P segments
+// of L points; a thread owns one segment of one line.  The two recurrences
+// of each solve run locally per segment with a zero carry-in; the carries
+// are then chained through LDS.  The chain stops K segments back: the factor
+// a carry picks up across a segment is the product of that segment's
+// elimination factors, and K is chosen on the host so that the product over
+// any K consecutive segments is below the precision's resolution (the terms
+// dropped are smaller than the rounding of the full chain).
+template <typename T, int L, int NR>
+__device__ __forceinline__ void seg_solve(T (&v)[NR][L], const int e0, const T *__restrict__ cs, const T alfa,
+                                          const T den, T *cf, T *cb, T *gf, T *gb, T *x0s, T *xns, const int line,
+                                          const int seg, const int TL, const int P, const int K) {
+  // NR right-hand sides share the matrix: each coefficient is read once.
+  const int PT = P * TL;
+  // forward elimination, zero carry-in
+  T prev[NR], g = T(1);
+#pragma unroll
+  for (int q = 0; q < NR; ++q) prev[q] = T(0);
+#pragma unroll
+  for (int m = 0; m < L; ++m) {
+    const T f = cs[4 * (e0 + m)];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      prev[q] = v[q][m] - prev[q] * f;
+      v[q][m] = prev[q];
+    }
+    g = -f * g;
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) cf[q * PT + seg * TL + line] = prev[q];
+  if (line == 0) gf[seg] = g;
+  __syncthreads();
+  if (seg > 0) {
+    T cin[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) cin[q] = T(0);
+    for (int s = seg - K > 0 ? seg - K : 0; s < seg; ++s) {
+      const T gs = gf[s];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) cin[q] = cf[q * PT + s * TL + line] + gs * cin[q];
+    }
+    T gg = T(1);
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      gg = -cs[4 * (e0 + m)] * gg;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) v[q][m] += gg * cin[q];
+    }
+  }
+  // back substitution, zero carry-in
+  T nxt[NR], h = T(1);
+#pragma unroll
+  for (int q = 0; q < NR; ++q) nxt[q] = T(0);
+#pragma unroll
+  for (int m = L - 1; m >= 0; --m) {
+    const T w = cs[4 * (e0 + m) + 1], ffw = cs[4 * (e0 + m) + 2];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      nxt[q] = w * v[q][m] - ffw * nxt[q];
+      v[q][m] = nxt[q];
+    }
+    h = -ffw * h;
+  }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) cb[q * PT + seg * TL + line] = nxt[q];
+  if (line == 0) gb[seg] = h;
+  __syncthreads();
+  if (seg < P - 1) {
+    T c[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) c[q] = T(0);
+    for (int s = seg + K < P - 1 ? seg + K : P - 1; s > seg; --s) {
+      const T gs = gb[s];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) c[q] = cb[q * PT + s * TL + line] + gs * c[q];
+    }
+    T hh = T(1);
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) {
+      hh = -cs[4 * (e0 + m) + 2] * hh;
+#pragma unroll
+      for (int q = 0; q < NR; ++q) v[q][m] += hh * c[q];
+    }
+  }
+  // periodic closure (Sherman-Morrison with the precomputed vector sm)
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    if (seg == 0) x0s[q * TL + line] = v[q][0];
+    if (seg == P - 1) xns[q * TL + line] = v[q][L - 1];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const T sx = (x0s[q * TL + line] - alfa * xns[q * TL + line]) / den;
+#pragma unroll
+    for (int m = 0; m < L; ++m) v[q][m] = v[q][m] - sx * cs[4 * (e0 + m) + 3];
+  }
+  // No barrier needed before the next solve: it writes cf/gf before its
+  // first barrier (last read here before the second), cb/gb after its first
+  // barrier (last read here before the third) and x0s/xns after its second.
+}
+
+template <typename T>
+__device__ __forceinline__ T &at(T *p, unsigned byte_off) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(p) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ const T &at(const T *p, unsigned byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + byte_off);
+}
+
+// ---------------------------------------------------------------- x wavefront solve
+// floor(a / b) for b > 0 (compile-time after unrolling)
+constexpr int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+template <typename T, int V, int HL, int HR>
+__device__ __forceinline__ void halo_window(const T (&own)[V], T (&v)[V + HL + HR], const int lane) {
+#pragma unroll
+  for (int k = 0; k < V + HL + HR; ++k) {
+    const int d = k - HL;
+    if (d >= 0 && d < V) {
+      v[k] = own[d];
+    } else {
+      const int lo = floor_div(d, V);
+      const int el = d - lo * V;
+      v[k] = __shfl(own[el], (lane + lo) & 63);
+    }
+  }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void wave_solve(T (&v)[V], const T (&fs)[V], const T (&fw)[V], const T (&ffw)[V],
+                                           const T (&sm)[V], const T alfa, const T den, const int lane) {
+  T prev = T(0), g = T(1);
+#pragma unroll
+  for (int m = 0; m < V; ++m) {
+    prev = v[m] - prev * fs[m];
+    v[m] = prev;
+    g = -fs[m] * g;
+  }
+  {
+    T a = g, bb = prev;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const T ap = __shfl_up(a, off), bp = __shfl_up(bb, off);
+      if (lane >= off) {
+        bb = a * bp + bb;
+        a = a * ap;
+      }
+    }
+    T cin = __shfl_up(bb, 1);
+    if (lane == 0) cin = T(0);
+    T gg = T(1);
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      gg = -fs[m] * gg;
+      v[m] += gg * cin;
+    }
+  }
+  T nxt = T(0), h = T(1);
+#pragma unroll
+  for (int m = V - 1; m >= 0; --m) {
+    nxt = fw[m] * v[m] - ffw[m] * nxt;
+    v[m] = nxt;
+    h = -ffw[m] * h;
+  }
+  {
+    T a = h, bb = nxt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const T an = __shfl_down(a, off), bn = __shfl_down(bb, off);
+      if (lane + off < 64) {
+        bb = a * bn + bb;
+        a = a * an;
+      }
+    }
+    T c = __shfl_down(bb, 1);
+    if (lane == 63) c = T(0);
+    T hh = T(1);
+#pragma unroll
+    for (int m = V - 1; m >= 0; --m) {
+      hh = -ffw[m] * hh;
+      v[m] += hh * c;
+    }
+  }
+  const T x0 = __shfl(v[0], 0), xn = __shfl(v[V - 1], 63);
+  const T sx = (x0 - alfa * xn) / den;
+#pragma unroll
+  for (int m = 0; m < V; ++m) v[m] = v[m] - sx * sm[m];
+}
+
+
+// Launch shape of a segmented y/z sweep for a line of n points: segments of
+// L = 8 (FP64) or 16 (FP32) points, so that the fused sweeps' three solves
+// keep their windows, right-hand sides and results within 128 VGPRs; TL
+// lines per block with TL*P <= 1024 threads and the LDS within 64 KiB --
+// for the fused sweeps two coefficient tables and the carries of two
+// right-hand sides, for a single operator (single = true) one of each.
+template <typename T>
+struct YzShape {
+  static constexpr int L = sizeof(T) == 8 ? 8 : 16;
+  int P = 0, TL = 0;
+  size_t shmem = 0;
+  static size_t lds(int n, int P, int TL, bool single) {
+    const size_t k = single ? 1 : 2;
+    return sizeof(T) * (4 * k * (size_t)n + 2 * k * (size_t)P * TL + 2 * (size_t)P + 2 * k * (size_t)TL);
+  }
+  bool plan(int n, bool single = false) {
+    if (n < 8 || n % L != 0) return false;
+    P = n / L;
+    // 128-byte rows (16 FP64 / 32 FP32 lines): full cache lines per point,
+    // and at 256-point lines 512-thread blocks, two of which share a CU
+    TL = 128 / (int)sizeof(T);
+    while (TL >= 8 && (TL * P > 1024 || lds(n, P, TL, single) > 65536)) TL >>= 1;
+    if (TL < 8) return false;
+    shmem = lds(n, P, TL, single);
+    return true;
+  }
+};
+
 // ---------------------------------------------------------------- kernel
 // AX: 0 x (LDS-staged), 1 y, 2 z (direct); EPI: fused product input and
 // epilogue present (a plain derivative compiles without them).
@@ -435,6 +656,85 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
   }
 }
 
+// ---------------------------------------------------------------- periodic y/z lines, segmented
+// One operator on periodic y or z lines (n_in == n_out == P*L): block of TL
+// lines (128-byte rows) x P segments of L points, every input read once into
+// a window, the solve of seg_solve with the set's truncated carry chain, and
+// the epilogue's inputs read before the first store (acc may be t).  Byte
+// offsets are 32-bit (the host checks the field size).
+template <typename T, int OP, int L, bool EPI>
+__global__ void __launch_bounds__(1024) seg_line_kernel(const LineArgs<T> A, const int kdepth) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  const int TL = blockDim.x, P = blockDim.y;
+  const int line = threadIdx.x, seg = threadIdx.y;
+  const int tid = seg * TL + line, nthr = TL * P;
+  const int n = A.n_out;
+  T *cs = smem, *cf = cs + 4 * n, *cb = cf + P * TL, *gf = cb + P * TL, *gb = gf + P, *x0s = gb + P,
+    *xns = x0s + TL;
+  for (int i = tid; i < 4 * n; i += nthr) cs[i] = A.pk[i];
+  __syncthreads();
+  const int a = blockIdx.x * TL + line, b = blockIdx.y;
+  const bool active = a < A.na;
+  const int e0 = seg * L;
+  const unsigned base = ((unsigned)a * (unsigned)A.sa_in + (unsigned)b * (unsigned)A.sb_in) * (unsigned)sizeof(T);
+  const unsigned step = (unsigned)A.se_in * (unsigned)sizeof(T);
+  auto wrap = [&](int j) { return (unsigned)(j < 0 ? j + n : (j >= n ? j - n : j)); };
+  T win[L + 8];
+#pragma unroll
+  for (int k = 0; k < L + 8; ++k) {
+    T x = T(0);
+    if (active) {
+      const unsigned o = base + wrap(e0 - 4 + k) * step;
+      x = at(A.u, o);
+      if (EPI && A.u2) x *= at(A.u2, o);
+    }
+    win[k] = x;
+  }
+  T w[1][L];
+#pragma unroll
+  for (int m = 0; m < L; ++m) w[0][m] = interior_row<OP, T, L + 8>(win, m + 4, A.s);
+  seg_solve<T, L, 1>(w, e0, cs, A.alfa, A.sm_den, cf, cb, gf, gb, x0s, xns, line, seg, TL, P, kdepth);
+  if (!active) return;
+  if (A.pp) {
+#pragma unroll
+    for (int m = 0; m < L; ++m) w[0][m] *= A.pp[e0 + m];
+  }
+  if constexpr (EPI) {
+    T mm[L], aa[L];
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      const unsigned o = base + (unsigned)(e0 + m) * step;
+      mm[m] = A.mul ? at(A.mul, o) : T(1);
+      aa[m] = A.acc ? at(A.acc, o) : T(0);
+    }
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      T r = w[0][m];
+      if (A.mul) r *= mm[m];
+      r = A.rc * r;
+      if (A.acc) r = A.ac * aa[m] + r;
+      at(A.t, base + (unsigned)(e0 + m) * step) = r;
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < L; ++m) at(A.t, base + (unsigned)(e0 + m) * step) = w[0][m];
+  }
+}
+
+template <typename T, int OP>
+static int launch_seg_line(const LineArgs<T> &a, const YzShape<T> &Y, bool epi, int kdepth, hipStream_t st) {
+  // the segment solve needs one line per block row and every segment full
+  const dim3 grid((a.na + Y.TL - 1) / Y.TL, a.nb), block(Y.TL, Y.P);
+  const size_t shmem = Y.shmem;
+  constexpr int L = YzShape<T>::L;
+  if (epi) hipLaunchKernelGGL((seg_line_kernel<T, OP, L, true>), grid, block, shmem, st, a, kdepth);
+  else hipLaunchKernelGGL((seg_line_kernel<T, OP, L, false>), grid, block, shmem, st, a, kdepth);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("segmented line launch: ") + hipGetErrorString(e));
+  return X3D_OK;
+}
+
 // ---------------------------------------------------------------- x lines, one wavefront per line
 // Periodic x lines of n = 64*V points: lane k owns points [kV, kV+V) of the
 // line (contiguous, so a wavefront reads/writes the line as one 64*V-word
@@ -478,7 +778,6 @@ __device__ __forceinline__ void store_run(T *__restrict__ p, const T (&in)[V]) {
   }
 }
 
-constexpr int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
 template <typename T, int OP, int V, bool EPI>
 __global__ void __launch_bounds__(256) xwave_kernel(const LineArgs<T> A, int nlines) {
@@ -497,15 +796,24 @@ __global__ void __launch_bounds__(256) xwave_kernel(const LineArgs<T> A, int nli
     ffw[m] = A.pk[4 * e + 2];
     sm[m] = A.pk[4 * e + 3];
   }
+  // the next line's input is loaded while the current one is solved
+  T nxt_u[V], nxt_u2[V];
+  if (wave < nlines) {
+    load_run<T, V>(A.u + (long long)wave * n + e0, nxt_u);
+    if (EPI && A.u2) load_run<T, V>(A.u2 + (long long)wave * n + e0, nxt_u2);
+  }
   for (int ln = wave; ln < nlines; ln += nwaves) {
     const long long base = (long long)ln * n + e0;
     T own[V];
-    load_run<T, V>(A.u + base, own);
-    if (EPI && A.u2) {
-      T o2[V];
-      load_run<T, V>(A.u2 + base, o2);
 #pragma unroll
-      for (int m = 0; m < V; ++m) own[m] *= o2[m];
+    for (int m = 0; m < V; ++m) own[m] = nxt_u[m];
+    if (EPI && A.u2) {
+#pragma unroll
+      for (int m = 0; m < V; ++m) own[m] *= nxt_u2[m];
+    }
+    if (ln + nwaves < nlines) {
+      load_run<T, V>(A.u + base + (long long)nwaves * n, nxt_u);
+      if (EPI && A.u2) load_run<T, V>(A.u2 + base + (long long)nwaves * n, nxt_u2);
     }
     // window U(e0-HL .. e0+V+HR-1): own points plus neighbour lanes' points
     T v[W];
@@ -731,6 +1039,18 @@ int launch_line(const LineCall<T> &c, hipStream_t st) {
       case OP_INTERVP: return launch_xwave<T, OP_INTERVP>(a, epi, nlines, st);
       case OP_DERPV: return launch_xwave<T, OP_DERPV>(a, epi, nlines, st);
       case OP_INTERPV: return launch_xwave<T, OP_INTERPV>(a, epi, nlines, st);
+    }
+  }
+  YzShape<T> Y;
+  if (!xmode && periodic && Y.plan(n_out, true) && (size_t)nx * ny * nz * sizeof(T) < ((size_t)1 << 32)) {
+    const int K = c.tri.kdepth > 0 ? c.tri.kdepth : Y.P;
+    switch (c.op) {
+      case OP_DER1: return launch_seg_line<T, OP_DER1>(a, Y, epi, K, st);
+      case OP_DER2: return launch_seg_line<T, OP_DER2>(a, Y, epi, K, st);
+      case OP_DERVP: return launch_seg_line<T, OP_DERVP>(a, Y, epi, K, st);
+      case OP_INTERVP: return launch_seg_line<T, OP_INTERVP>(a, Y, epi, K, st);
+      case OP_DERPV: return launch_seg_line<T, OP_DERPV>(a, Y, epi, K, st);
+      case OP_INTERPV: return launch_seg_line<T, OP_INTERPV>(a, Y, epi, K, st);
     }
   }
   const int ax = c.dir;

@@ -1,6 +1,7 @@
 // Host FP64 computation of every coefficient set of schemes.f90 and upload of
 // FP64/FP32 device copies.  The arithmetic follows schemes.f90 statement by
 // statement so that the arrays equal the reference's module arrays.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -266,15 +267,52 @@ void cast_to(const Sten<double> &a, Sten<T> &b) {
 TriDev<double> TriSet::dev64() const {
   TriDev<double> d;
   d.pk = d64;
-  d.alfa = alfa; d.sm_den = sm_den; d.n = (int)ff.size(); d.periodic = periodic;
+  d.alfa = alfa; d.sm_den = sm_den; d.n = (int)ff.size(); d.periodic = periodic; d.kdepth = kd8;
   return d;
 }
 
 TriDev<float> TriSet::dev32() const {
   TriDev<float> d;
   d.pk = d32;
-  d.alfa = (float)alfa; d.sm_den = (float)sm_den; d.n = (int)ff.size(); d.periodic = periodic;
+  d.alfa = (float)alfa; d.sm_den = (float)sm_den; d.n = (int)ff.size(); d.periodic = periodic; d.kdepth = kd16;
   return d;
+}
+
+bool same_set(const TriSet &a, const TriSet &b) {
+  return &a == &b || (a.ff == b.ff && a.fs == b.fs && a.fw == b.fw && a.sm == b.sm && a.alfa == b.alfa &&
+                      a.sm_den == b.sm_den);
+}
+
+// Smallest K such that the elimination factors of any K consecutive segments
+// of L points multiply to less than tol*(1-rho) in magnitude, rho being the
+// largest single-segment product: the carries of segments further back then
+// change no result by more than tol times the largest carry.  Forward factors
+// are fs, backward factors ff*fw (the recurrences of prepare, schemes.f90:
+// 413-439).  Returns P (the full chain) when no shorter depth qualifies.
+int chain_depth(const TriSet &s, int L, double tol) {
+  const int n = (int)s.fs.size();
+  const int P = (n + L - 1) / L;
+  std::vector<double> gfw(P, 1.0), gbw(P, 1.0);
+  for (int e = 0; e < n; ++e) {
+    gfw[e / L] *= std::fabs(s.fs[e]);
+    gbw[e / L] *= std::fabs(s.ff[e] * s.fw[e]);
+  }
+  double rho = 0;
+  for (int p = 0; p < P; ++p) rho = std::max(rho, std::max(gfw[p], gbw[p]));
+  if (rho >= 1.0) return P;
+  for (int K = 1; K < P; ++K) {
+    double worst = 0;
+    for (int p0 = 0; p0 + K <= P; ++p0) {
+      double pf = 1, pb = 1;
+      for (int q = p0; q < p0 + K; ++q) {
+        pf *= gfw[q];
+        pb *= gbw[q];
+      }
+      worst = std::max(worst, std::max(pf, pb));
+    }
+    if (worst < tol * (1.0 - rho)) return K;
+  }
+  return P;
 }
 
 int upload_set(TriSet &s) {
@@ -292,6 +330,10 @@ int upload_set(TriSet &s) {
   X3D_HIP(hipMalloc(&s.d32, 4 * n * sizeof(float)));
   X3D_HIP(hipMemcpy(s.d64, h.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
   X3D_HIP(hipMemcpy(s.d32, f.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
+  // carry-chain depths of the segmented y/z solvers (8-point segments in
+  // FP64, 16-point in FP32)
+  s.kd8 = chain_depth(s, 8, 1e-18);
+  s.kd16 = chain_depth(s, 16, 1e-9);
   return X3D_OK;
 }
 

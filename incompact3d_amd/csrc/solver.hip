@@ -27,6 +27,7 @@ int launch_ab(double *u, const H *f1, const H *f2, const H *f3, double a, double
 template <typename P>
 int launch_corvel(double *u, const P *p, float *shadow, size_t n, hipStream_t st);
 int launch_cast(const double *a, float *b, size_t n, hipStream_t st);
+int launch_widen(const float *a, double *b, size_t n, hipStream_t st);
 int launch_tgv(double *ux, double *uy, double *uz, int nx, int ny, int nz, double dx, double dy, double dz,
                hipStream_t st);
 template <typename T>
@@ -54,6 +55,13 @@ struct x3d_solver {
   double *w64[4] = {};         // FP64 work for the divergence check of precision 1
   double *partial = nullptr;   // reduction partials
   double *red = nullptr;       // reduction results (device)
+  // LES (les_models.f90): 0 none, 1 Smagorinsky (jles=1), 2 learned closure,
+  // 3 its eddy-viscosity projection
+  int les = 0, iconserv = 0;
+  double cs = 0.15, beta = 0.0, gscale = 1.0, delta0 = 1.0;
+  x3d::MlpDev *mlp = nullptr;  // not owned
+  void *lesw = nullptr;        // 20 fields: g[9], s or T [6], nut, ta, sgs[3]
+  int *nfb = nullptr;          // fallback counter of the learned closure (device)
 };
 
 namespace x3d {
@@ -100,6 +108,68 @@ int momentum_component(const Context &c, const T *const u[3], int comp, T *W, T 
       X3D_TRY(line<T>(c, d, OP_DER1, pv, uc, nullptr, W, W, u[d], 1.0, -0.5, nx, ny, nz));
       T *dst = d == 0 ? out : W;
       X3D_TRY(line<T>(c, d, OP_DER2, pv, uc, nullptr, dst, W, nullptr, 1.0, xnu, nx, ny, nz));
+    }
+  }
+  return X3D_OK;
+}
+
+// ---------------------------------------------------------------- LES (les_models.f90)
+// SGS force of the current velocity into sgs[3] (compute precision):
+// Smagorinsky in the nonconservative form (sgs_mom_nonconservative,
+// les_models.f90:1091-1250) or the conservative one (:1327-1456), or the
+// learned closure T_ij followed by d_j T_ij with the conservative parities.
+template <typename T>
+int stress_divergence(const Context &c, T *const t[6], T *const sgs[3], int nx, int ny, int nz) {
+  // les_models.f90:1384-1440; t = xx yy zz xy xz yz
+  X3D_TRY(line<T>(c, 0, OP_DER1, 0, t[0], nullptr, sgs[0], nullptr, nullptr, 0, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 0, OP_DER1, 0, t[3], nullptr, sgs[1], nullptr, nullptr, 0, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 0, OP_DER1, 0, t[4], nullptr, sgs[2], nullptr, nullptr, 0, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 1, OP_DER1, 0, t[3], nullptr, sgs[0], sgs[0], nullptr, 1, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 1, OP_DER1, 1, t[1], nullptr, sgs[1], sgs[1], nullptr, 1, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 1, OP_DER1, 0, t[5], nullptr, sgs[2], sgs[2], nullptr, 1, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 2, OP_DER1, 0, t[4], nullptr, sgs[0], sgs[0], nullptr, 1, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 2, OP_DER1, 0, t[5], nullptr, sgs[1], sgs[1], nullptr, 1, 1, nx, ny, nz));
+  X3D_TRY(line<T>(c, 2, OP_DER1, 0, t[2], nullptr, sgs[2], sgs[2], nullptr, 1, 1, nx, ny, nz));
+  return X3D_OK;
+}
+
+template <typename T>
+int compute_sgs(const Context &c, x3d_solver *S, const T *const u[3], T *const sgs[3]) {
+  const int nx = S->nx, ny = S->ny, nz = S->nz;
+  const size_t n = S->n;
+  T *f = (T *)S->lesw;
+  T *g[9], *s6[6];
+  for (int i = 0; i < 9; ++i) g[i] = f + i * n;
+  for (int i = 0; i < 6; ++i) s6[i] = f + (9 + i) * n;
+  T *nut = f + 15 * n, *ta = f + 16 * n;
+  // the nine gradients with the parities of smag (les_models.f90:210-238):
+  // g[3*d + comp] = d u_comp / d x_d, parity 0 when d == comp
+  for (int d = 0; d < 3; ++d)
+    for (int comp = 0; comp < 3; ++comp)
+      X3D_TRY(line<T>(c, d, OP_DER1, d == comp ? 0 : 1, u[comp], nullptr, g[3 * d + comp], nullptr, nullptr, 0, 1,
+                      nx, ny, nz));
+  const T *cg[9];
+  for (int i = 0; i < 9; ++i) cg[i] = g[i];
+  const double delta = std::cbrt(c.dir[0].d * c.dir[1].d * c.dir[2].d);   // les_models.f90:36-39
+  if (S->les >= 2) {
+    X3D_TRY(launch_closure<T>(S->mlp, cg, s6, n, S->gscale, delta / S->delta0, delta, S->cs, S->beta,
+                              S->les == 3 ? 1 : 0, S->nfb, c.stream));
+    return stress_divergence<T>(c, s6, sgs, nx, ny, nz);
+  }
+  X3D_TRY(launch_strain_nut<T>(cg, s6, nut, (S->cs * delta) * (S->cs * delta), n, c.stream));
+  if (S->iconserv == 1) {
+    X3D_TRY(launch_smag_stress<T>(nut, s6, s6, n, c.stream));
+    return stress_divergence<T>(c, s6, sgs, nx, ny, nz);
+  }
+  // s6 = xx yy zz xy xz yz; S_{comp,d} for the 2 S_ij d_j nut term
+  const int sidx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+  for (int d = 0; d < 3; ++d) {
+    X3D_TRY(line<T>(c, d, OP_DER1, 1, nut, nullptr, ta, nullptr, nullptr, 0, 1, nx, ny, nz));
+    for (int comp = 0; comp < 3; ++comp) {
+      // sgs = sgs + nut * d2_d u (d == x: sgs = d2 u * nut)
+      X3D_TRY(line<T>(c, d, OP_DER2, d == comp ? 0 : 1, u[comp], nullptr, sgs[comp], d == 0 ? nullptr : sgs[comp],
+                      nut, d == 0 ? 0.0 : 1.0, 1.0, nx, ny, nz));
+      X3D_TRY(launch_fma3<T>(sgs[comp], s6[sidx[comp][d]], ta, 2.0, n, c.stream));
     }
   }
   return X3D_OK;
@@ -166,12 +236,18 @@ int step_once(Context &c, x3d_solver *S) {
   // transeq (z, y, x sweeps per component) with AB3 fused into the x sweep;
   // the new velocity goes to the second buffer set
   const T *cu[3] = {u[0], u[1], u[2]};
+  // SGS force (transeq.f90:498-503), added to the RHS before AB3
+  T *sgs[3] = {nullptr, nullptr, nullptr};
+  if (S->les) {
+    for (int i = 0; i < 3; ++i) sgs[i] = (T *)S->lesw + (17 + i) * S->n;
+    X3D_TRY(compute_sgs<T>(c, S, cu, sgs));
+  }
   if (S->fused) {
     for (int comp = 0; comp < 3; ++comp) {
       T *f1 = (T *)S->hist[comp][S->slot[0]];
       const T *f2 = (T *)S->hist[comp][S->slot[1]];
       const T *f3 = (T *)S->hist[comp][S->slot[2]];
-      X3D_TRY(launch_momentum<T>(c, 2, comp, cu, nullptr, W[0], nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0,
+      X3D_TRY(launch_momentum<T>(c, 2, comp, cu, sgs[comp], W[0], nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0,
                                  0, 0, nx, ny, nz, c.stream));
       X3D_TRY(launch_momentum<T>(c, 1, comp, cu, W[0], W[0], nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0,
                                  0, nx, ny, nz, c.stream));
@@ -186,8 +262,11 @@ int step_once(Context &c, x3d_solver *S) {
       cu[i] = u[i];
     }
   } else {
-    for (int comp = 0; comp < 3; ++comp)
-      X3D_TRY(momentum_component<T>(c, cu, comp, W[0], (T *)S->hist[comp][S->slot[0]], nx, ny, nz));
+    for (int comp = 0; comp < 3; ++comp) {
+      T *f1 = (T *)S->hist[comp][S->slot[0]];
+      X3D_TRY(momentum_component<T>(c, cu, comp, W[0], f1, nx, ny, nz));
+      if (S->les) X3D_TRY(launch_add<T>(f1, sgs[comp], S->n, c.stream));
+    }
     for (int comp = 0; comp < 3; ++comp) {
       const T *f1 = (T *)S->hist[comp][S->slot[0]];
       const T *f2 = (T *)S->hist[comp][S->slot[1]];
@@ -307,6 +386,8 @@ void free_solver(x3d_solver *S) {
     if (S->w64[i]) (void)hipFree(S->w64[i]);
   }
   if (S->diag) (void)hipFree(S->diag);
+  if (S->lesw) (void)hipFree(S->lesw);
+  if (S->nfb) (void)hipFree(S->nfb);
   if (S->partial) (void)hipFree(S->partial);
   if (S->red) (void)hipFree(S->red);
 }
@@ -434,6 +515,68 @@ int x3d_solver_divergence_stats(x3d_solver *S, double out2[2]) {
   X3D_TRY(launch_max_mean<double>(W[0], np, S->partial, S->red, c->stream));
   X3D_HIP(hipMemcpyAsync(out2, S->red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   X3D_HIP(hipStreamSynchronize(c->stream));
+  return X3D_OK;
+}
+
+int x3d_solver_set_les(x3d_solver *S, int model, double cs, int iconserv, double beta, x3d_mlp *mlp, double gscale,
+                       double delta0) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S) return fail(X3D_ERR_ARG, "null solver");
+  if (model < 0 || model > 3)
+    return fail(X3D_ERR_ARG, "LES model must be 0 (none), 1 (Smagorinsky), 2 or 3 (learned)");
+  if (iconserv != 0 && iconserv != 1) return fail(X3D_ERR_ARG, "iconserv must be 0 or 1");
+  if (model >= 2 && (!mlp || !(gscale > 0) || !(delta0 > 0)))
+    return fail(X3D_ERR_ARG, "learned closure needs a network and positive gscale, delta0");
+  if (!(cs >= 0)) return fail(X3D_ERR_ARG, "cs must be >= 0");
+  if (model && !S->lesw) {
+    const size_t es = S->precision == 0 ? 8 : 4;
+    X3D_HIP(hipMalloc(&S->lesw, 20 * S->n * es));
+    X3D_HIP(hipMalloc(&S->nfb, sizeof(int)));
+    X3D_HIP(hipMemset(S->nfb, 0, sizeof(int)));
+  }
+  S->les = model;
+  S->cs = cs;
+  S->iconserv = iconserv;
+  S->beta = beta;
+  S->mlp = model >= 2 ? reinterpret_cast<MlpDev *>(mlp) : nullptr;
+  S->gscale = gscale;
+  S->delta0 = delta0;
+  return X3D_OK;
+}
+
+int x3d_solver_sgs(x3d_solver *S, double *sx, double *sy, double *sz) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || !sx || !sy || !sz) return fail(X3D_ERR_ARG, "null pointer");
+  if (!S->les) return fail(X3D_ERR_STATE, "no LES model set");
+  double *out[3] = {sx, sy, sz};
+  if (S->precision == 0) {
+    const double *u[3] = {S->u64[0], S->u64[1], S->u64[2]};
+    X3D_TRY(compute_sgs<double>(*c, S, u, out));
+  } else {
+    const float *u[3] = {S->u32[0], S->u32[1], S->u32[2]};
+    float *sg[3];
+    for (int i = 0; i < 3; ++i) sg[i] = (float *)S->lesw + (17 + i) * S->n;
+    X3D_TRY(compute_sgs<float>(*c, S, u, sg));
+    for (int i = 0; i < 3; ++i) X3D_TRY(launch_widen(sg[i], out[i], S->n, c->stream));
+  }
+  X3D_HIP(hipStreamSynchronize(c->stream));
+  return X3D_OK;
+}
+
+int x3d_solver_les_fallbacks(x3d_solver *S, long long *count) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || !count) return fail(X3D_ERR_ARG, "null pointer");
+  int h = 0;
+  if (S->nfb) {
+    X3D_HIP(hipMemcpyAsync(&h, S->nfb, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    X3D_HIP(hipStreamSynchronize(c->stream));
+    X3D_HIP(hipMemsetAsync(S->nfb, 0, sizeof(int), c->stream));
+    X3D_HIP(hipStreamSynchronize(c->stream));
+  }
+  *count = h;
   return X3D_OK;
 }
 

@@ -56,6 +56,7 @@ struct TriDev {
   T alfa = 0, sm_den = 1;
   int n = 0;
   int periodic = 0;
+  int kdepth = 0;               // segment carry-chain depth (0: full chain)
 };
 
 // Host FP64 description of a tridiagonal set, with device FP64/FP32 copies.
@@ -66,12 +67,17 @@ struct TriSet {
   int periodic = 0;
   double *d64 = nullptr;        // packed {fs, fw, ff*fw, r} x n, FP64
   float *d32 = nullptr;         // the same, FP32
+  int kd8 = 0, kd16 = 0;        // chain depths for 8- and 16-point segments
   TriDev<double> dev64() const;
   TriDev<float> dev32() const;
 };
 
 // Upload the packed device copies of one set (allocates d64/d32).
 int upload_set(TriSet &s);
+// Same coefficients (ff, fs, fw, sm, alfa, denominator).
+bool same_set(const TriSet &a, const TriSet &b);
+// Segment carry-chain depth for L-point segments at tolerance tol.
+int chain_depth(const TriSet &s, int L, double tol);
 
 struct DirCoef {
   int n = 0, nm = 0, bc1 = 0, bcn = 0;
@@ -168,5 +174,39 @@ bool projection_fusable(const Context &c);
 template <typename T>
 int launch_projection(const Context &c, int sw, const T *const in[3], T *const out[3], double *const u64[3],
                       float *const u32[3], int nx, int ny, int nz, hipStream_t st);
+
+// ---------------------------------------------------------------- closure network (mlp.hip)
+struct MlpDev;
+constexpr int MLP_PARAMS = 9414;   // W1 b1 W2 b2 W3 b3 W4 b4 of 10-64-64-64-6
+int mlp_alloc(MlpDev *&out);
+void mlp_free(MlpDev *M);
+int mlp_set_params(MlpDev *M, const float *host, hipStream_t st);
+int mlp_get_params(const MlpDev *M, float *host, hipStream_t st);
+int mlp_forward(const MlpDev *M, const float *x, float *y, int n, hipStream_t st);
+int mlp_train_batch(MlpDev *M, const float *x, const float *t, int n, int apply, float lr, float *loss_host,
+                    hipStream_t st);
+int mlp_get_grad(const MlpDev *M, float *host, hipStream_t st);
+// T_ij of the network (with the Smagorinsky fallback) from the nine gradients
+template <typename T>
+int launch_closure(const MlpDev *M, const T *const g[9], T *const t[6], size_t n, double gscale, double dfeat,
+                   double delta, double cs, double beta, int project, int *nfallback, hipStream_t st);
+
+// ---------------------------------------------------------------- LES fields (les.hip)
+// strain S_ij (xx yy zz xy xz yz) and Smagorinsky nut from the nine gradients
+template <typename T>
+int launch_strain_nut(const T *const g[9], T *const s[6], T *nut, double cs_d2, size_t n, hipStream_t st);
+// acc_c += a * x * y (elementwise, FP64 arithmetic)
+template <typename T>
+int launch_fma3(T *acc, const T *x, const T *y, double a, size_t n, hipStream_t st);
+// y += x
+template <typename T>
+int launch_add(T *y, const T *x, size_t n, hipStream_t st);
+// t_ij = 2 nut S_ij
+template <typename T>
+int launch_smag_stress(const T *nut, const T *const s[6], T *const t[6], size_t n, hipStream_t st);
+// top-hat filter of a fine (nx,ny,nz) velocity onto the (nx/f,ny/f,nz/f)
+// grid: filtered u_i and tau_ij = bar(u_i u_j) - bar(u_i) bar(u_j)
+int launch_box_filter(const double *const u[3], int nx, int ny, int nz, int f, double *const ub[3],
+                      double *const tau[6], hipStream_t st);
 
 }  // namespace x3d

@@ -10,7 +10,9 @@ Writes tests/golden/tgv_reference.json with
     of eps_t;
   * the full tests/TGV-Taylor-Green-vortex/reference_time_evol.dat (CTest
     fixture: 65^3 free-slip RK3 run, columns t, E_k, eps, eps2, enstrophy)
-    and its single-precision twin.
+    and its single-precision twin;
+  * the t, E_k, eps_t columns of examples/TGV-Taylor-Green-vortex/
+    TGV_Re5000.dat (Incompact3d DNS at 1280^3) for t <= 12 and its eps_t peak.
 """
 import json
 import os
@@ -35,7 +37,12 @@ def main(ref):
     peak = max(dns, key=lambda r: r[2])
     ctest = read_table(os.path.join(ref, "tests/TGV-Taylor-Green-vortex/reference_time_evol.dat"))
     ctest1 = read_table(os.path.join(ref, "tests/TGV-Taylor-Green-vortex/reference_time_evol_single.dat"))
+    re5k = read_table(os.path.join(ref, "examples/TGV-Taylor-Green-vortex/TGV_Re5000.dat"))
+    peak5k = max(re5k, key=lambda r: r[2])
     out = {
+        "TGV_Re5000_dns1280": {"columns": ["t", "E_k", "eps_t"], "rows": [r[:3] for r in re5k if r[0] <= 12.0 + 1e-9],
+                               "peak_eps_t": peak5k[:3],
+                               "source": "examples/TGV-Taylor-Green-vortex/TGV_Re5000.dat"},
         "TGV_Re1600_dns512": {"columns": ["t", "E_k", "eps_t", "eps", "enstrophy"], "rows": sel,
                               "peak_eps_t": peak[:5],
                               "source": "examples/TGV-Taylor-Green-vortex/TGV_Re1600.dat"},
