@@ -197,6 +197,40 @@ def microbench(X, n, walls, stream, reps=10):
     return {"grid": [n, ny, n], "y": "Dirichlet walls (ncl=2)" if walls else "periodic", "ops": out}
 
 
+def learned_les_comparison(X, stream, fp64_ms_256, n_dns):
+    """Time to solution at equal dissipation error, TGV Re=5000 to t=12:
+    the FP64 plain solver on the 256^3 DNS grid against the FP32-bulk solver
+    with the learned closure on 64^3.  The closure is trained in this run on
+    the box-filtered DNS (incompact3d_amd.les.experiment); the errors are the
+    dissipation-peak errors against the DNS.  The FP64 step time is this
+    run's 256^3 measurement (Re changes nothing in the cost)."""
+    import time as _t
+    from incompact3d_amd import les
+    t0 = _t.perf_counter()
+    with __import__("torch").cuda.stream(stream):
+        out = les.experiment(les_precision=1, time_steps=200)
+    wall = _t.perf_counter() - t0
+    dt_dns, dt_les, t_end = out["dns"]["dt"], out["les"]["dt"], out["dns"]["t_end"]
+    les_ms = out["les"]["ms_per_step_learned"]
+    dns_s = fp64_ms_256 * 1e-3 * t_end / dt_dns
+    les_s = les_ms * 1e-3 * t_end / dt_les
+    r = out["runs"]
+    return {
+        "workload": f"TGV Re=5000 to t={t_end:g}: FP64 plain {n_dns}^3 (dt {dt_dns}) vs FP32-bulk + learned closure "
+                    f"{out['les']['grid']}^3 (dt {dt_les})",
+        "fp64_plain_seconds_to_t": round(dns_s, 3),
+        "fp32_learned_les_seconds_to_t": round(les_s, 4),
+        "fp32_learned_les_ms_per_step": round(les_ms, 4),
+        "time_to_solution_speedup": round(dns_s / les_s, 1),
+        "peak_eps_dns": out["dns"]["peak_eps_dns"],
+        "peak_eps_err_vs_dns": {k: round(v["peak_rel_err_vs_dns"], 4) for k, v in r.items()},
+        "peak_eps_err_vs_filtered_dns": {k: round(v["peak_rel_err"], 4) for k, v in r.items()},
+        "eps_rms_err_vs_filtered_dns": {k: round(v["eps_rms_rel_err"], 4) for k, v in r.items()},
+        "training_points": out["training"]["points"],
+        "experiment_wall_seconds": round(wall, 1),
+    }
+
+
 def cpu_baseline(n, steps=1):
     """The oracle (numpy, single thread) on a bounded sample of the same
     workload: `steps` TGV time steps at n^3 after one untimed step."""
@@ -225,6 +259,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-les", action="store_true",
+                    help="skip the learned-LES time-to-solution comparison (TGV Re=5000, about 40 s)")
     ap.add_argument("--micro-n", type=int, default=512,
                     help="kernel microbench grid (0 = skip): n^3 periodic and n x (n+1) x n with y walls")
     ap.add_argument("--only", choices=["fp64", "fp32", "both"], default="both")
@@ -303,6 +339,8 @@ def main():
                                    "speedup_vs_fp64": round((npts / (les_ms * 1e-3) / 1e6) / value, 3)}
     out["device"] = X.device_name()
     X.finalize()
+    if not args.no_les and r64 is not None:
+        out["learned_les_equal_error"] = learned_les_comparison(X, stream, r64["event_ms_per_step"], n)
     if args.micro_n > 0:
         out["microbench"] = [microbench(X, args.micro_n, False, stream), microbench(X, args.micro_n, True, stream)]
     if not args.no_cpu_baseline:

@@ -38,12 +38,8 @@ constexpr int T_A1 = 0, T_A2 = 4, T_A3 = 20, T_A4 = 36, T_AT4 = 40, T_AT3 = 44, 
 constexpr int MLP_WG_BLOCKS = 256;   // blocks of the weight-gradient kernel
 
 // ---------------------------------------------------------------- BF16 helpers
-// round to nearest even (NaN inputs are not expected)
-__device__ __forceinline__ short to_bf16(float f) {
-  unsigned u = __float_as_uint(f);
-  u = (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-  return (short)u;
-}
+// round to nearest even: gfx950's v_cvt_pk_bf16_f32
+__device__ __forceinline__ short to_bf16(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 __device__ __forceinline__ float from_bf16(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
@@ -424,6 +420,10 @@ struct ClosureArgs {
   size_t n;
 };
 
+// A wavefront takes 64 points per iteration, one per lane: the nine
+// gradients are read once (coalesced), cross-lane reads build the four
+// 16-point MFMA input tiles, the outputs come back to the point's own lane,
+// and the fallback / projection epilogue runs on all 64 lanes.
 template <typename T>
 __global__ void __launch_bounds__(256) closure_kernel(const bf16x4 *__restrict__ tiles, const float *__restrict__ bias,
                                                      const ClosureArgs<T> A, int *__restrict__ nfallback) {
@@ -432,57 +432,67 @@ __global__ void __launch_bounds__(256) closure_kernel(const bf16x4 *__restrict__
   for (int i = threadIdx.x; i < 208; i += blockDim.x) sb[i] = bias[i];
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
-  const int lane = threadIdx.x & 63, kb = lane >> 4;
+  const int lane = threadIdx.x & 63, kb = lane >> 4, r16 = lane & 15;
   bf16x4 At[40];
 #pragma unroll
   for (int t = 0; t < 40; ++t) At[t] = tiles[t * 64 + lane];
   const long long wave = (blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 6;
   const long long nwaves = (gridDim.x * (long long)blockDim.x) >> 6;
-  const long long ntiles = (long long)((A.n + 15) / 16);
+  const long long nchunks = (long long)((A.n + 63) / 64);
   int local = 0;
-  for (long long tile = wave; tile < ntiles; tile += nwaves) {
-    const size_t pt = (size_t)tile * 16 + (lane & 15);
+  for (long long chunk = wave; chunk < nchunks; chunk += nwaves) {
+    const size_t pt = (size_t)chunk * 64 + lane;
     const bool valid = pt < A.n;
-    bf16x4 x;
+    T g[9];
+    float xf[9];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int f = 4 * kb + j;
-      float v = 0.f;
-      if (valid) {
-        if (f < 9) v = (float)A.g[f][pt] * A.inv_gscale;
-        else if (f == 9) v = A.dfeat;
-      }
-      x[j] = to_bf16(v);
+    for (int f = 0; f < 9; ++f) {
+      g[f] = valid ? A.g[f][pt] : T(0);
+      xf[f] = (float)g[f] * A.inv_gscale;
     }
-    Fwd16 F;
-    const f32x4 y = forward16(At, sb, x, kb, F);
-    // outputs 4, 5 live in the lanes 16 above
-    const float y4 = __shfl(y[0], (lane + 16) & 63), y5 = __shfl(y[1], (lane + 16) & 63);
-    if (kb == 0 && valid) {
-      double g[9];
+    float yo[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma nounroll
+    for (int t = 0; t < 4; ++t) {
+      // input tile t: column r16 is point 16t + r16, rows 4kb..4kb+3 its features
+      const int src = 16 * t + r16;
+      bf16x4 x;
 #pragma unroll
-      for (int f = 0; f < 9; ++f) g[f] = (double)A.g[f][pt];
+      for (int j = 0; j < 4; ++j) {
+        const float c0 = __shfl(xf[j], src), c1 = __shfl(xf[4 + j], src);
+        const float c2 = j == 0 ? __shfl(xf[8], src) : (j == 1 ? A.dfeat : 0.f);
+        x[j] = to_bf16(kb == 0 ? c0 : (kb == 1 ? c1 : (kb == 2 ? c2 : 0.f)));
+      }
+      Fwd16 F;
+      const f32x4 y = forward16(At, sb, x, kb, F);
+      // outputs 0..3 of column r16 sit in lane r16, outputs 4..5 in lane 16 + r16
+      const float o0 = __shfl(y[0], r16), o1 = __shfl(y[1], r16), o2 = __shfl(y[2], r16), o3 = __shfl(y[3], r16);
+      const float o4 = __shfl(y[0], 16 + r16), o5 = __shfl(y[1], 16 + r16);
+      if (kb == t) {
+        yo[0] = o0; yo[1] = o1; yo[2] = o2; yo[3] = o3; yo[4] = o4; yo[5] = o5;
+      }
+    }
+    if (valid) {
       // S_ij as les_models.f90:214-269
-      const double sxx = g[0], syy = g[4], szz = g[8];
-      const double sxy = 0.5 * (g[3] + g[1]), sxz = 0.5 * (g[2] + g[6]), syz = 0.5 * (g[7] + g[5]);
-      const double ss = sxx * sxx + syy * syy + szz * szz + 2.0 * sxy * sxy + 2.0 * sxz * sxz + 2.0 * syz * syz;
-      const double nut = A.cs_d2 * sqrt(2.0 * ss);
-      double t[6] = {y[0], y[1], y[2], y[3], y4, y5};
+      const T sxx = g[0], syy = g[4], szz = g[8];
+      const T sxy = T(0.5) * (g[3] + g[1]), sxz = T(0.5) * (g[2] + g[6]), syz = T(0.5) * (g[7] + g[5]);
+      const T ss = sxx * sxx + syy * syy + szz * szz + T(2) * sxy * sxy + T(2) * sxz * sxz + T(2) * syz * syz;
+      const T nut = (T)A.cs_d2 * sqrt(T(2) * ss);
+      T t[6];
 #pragma unroll
-      for (int c = 0; c < 6; ++c) t[c] *= (double)A.out_scale;
-      const double pi = t[0] * sxx + t[1] * syy + t[2] * szz + 2.0 * (t[3] * sxy + t[4] * sxz + t[5] * syz);
-      if (pi < -A.beta * 2.0 * nut * ss) {
-        t[0] = 2.0 * nut * sxx; t[1] = 2.0 * nut * syy; t[2] = 2.0 * nut * szz;
-        t[3] = 2.0 * nut * sxy; t[4] = 2.0 * nut * sxz; t[5] = 2.0 * nut * syz;
+      for (int c = 0; c < 6; ++c) t[c] = (T)yo[c] * (T)A.out_scale;
+      const T pi = t[0] * sxx + t[1] * syy + t[2] * szz + T(2) * (t[3] * sxy + t[4] * sxz + t[5] * syz);
+      if (pi < -(T)A.beta * T(2) * nut * ss) {
+        t[0] = T(2) * nut * sxx; t[1] = T(2) * nut * syy; t[2] = T(2) * nut * szz;
+        t[3] = T(2) * nut * sxy; t[4] = T(2) * nut * sxz; t[5] = T(2) * nut * syz;
         ++local;
       } else if (A.project) {
         // the part of T along S: an eddy viscosity nu_t = max(T:S, 0) / (2 S:S)
-        const double nt = (ss > 0.0 && pi > 0.0) ? pi / (2.0 * ss) : 0.0;
-        t[0] = 2.0 * nt * sxx; t[1] = 2.0 * nt * syy; t[2] = 2.0 * nt * szz;
-        t[3] = 2.0 * nt * sxy; t[4] = 2.0 * nt * sxz; t[5] = 2.0 * nt * syz;
+        const T nt = (ss > T(0) && pi > T(0)) ? pi / (T(2) * ss) : T(0);
+        t[0] = T(2) * nt * sxx; t[1] = T(2) * nt * syy; t[2] = T(2) * nt * szz;
+        t[3] = T(2) * nt * sxy; t[4] = T(2) * nt * sxz; t[5] = T(2) * nt * syz;
       }
 #pragma unroll
-      for (int c = 0; c < 6; ++c) A.t[c][pt] = (T)t[c];
+      for (int c = 0; c < 6; ++c) A.t[c][pt] = t[c];
     }
   }
   if (nfallback) {
@@ -608,8 +618,8 @@ int launch_closure(const MlpDev *M, const T *const g[9], T *const t[6], size_t n
   A.cs_d2 = (cs * delta) * (cs * delta);
   A.beta = beta;
   A.n = n;
-  const size_t ntiles = (n + 15) / 16;
-  const int blocks = (int)std::min<size_t>((ntiles + 3) / 4, 4096);
+  const size_t nchunks = (n + 63) / 64;
+  const int blocks = (int)std::min<size_t>((nchunks + 3) / 4, 4096);
   hipLaunchKernelGGL((closure_kernel<T>), dim3(blocks), dim3(256), 0, st, M->tiles, M->bias, A, nfallback);
   X3D_HIP(hipGetLastError());
   return X3D_OK;

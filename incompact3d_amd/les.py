@@ -42,6 +42,14 @@ class FilteredDNS:
     ke_dns: list = field(default_factory=list)
 
 
+def _init(X, p):
+    """x3d_init plus the library stream = torch's current stream, so that the
+    torch operations of this module and the library's kernels are ordered."""
+    import torch
+    X.init(p)
+    X.set_stream(torch.cuda.current_stream().cuda_stream)
+
+
 def _ke(ub):
     return float(sum((c * c).mean().item() for c in ub) * 0.5)
 
@@ -51,7 +59,7 @@ def dns_filtered(n=256, f=4, re=5000.0, dt=0.002, t_end=12.0, snap_times=(4, 5, 
     """Run the DNS and box-filter it (see the module docstring)."""
     import incompact3d_amd as X
     p = X.make_params(n, n, n, re=re, dt=dt)
-    X.init(p)
+    _init(X, p)
     out = FilteredDNS(n, f, re, dt, list(snap_times))
     try:
         s = X.Solver(p, precision)
@@ -216,8 +224,34 @@ def apriori(mlp, x, y_phys_scaled):
     return out
 
 
+def time_les_steps(n, re, dt, model, mlp=None, G=1.0, delta0=1.0, cs=0.17, beta=1.0, precision=1, steps=200):
+    """ms per step of the coarse LES (no diagnostics), HIP events on the
+    library stream; the coarse context must be initialised by _init."""
+    import torch
+    import incompact3d_amd as X
+    stream = torch.cuda.current_stream()   # the library stream (see _init)
+    p = X.make_params(n, n, n, re=re, dt=dt)
+    s = X.Solver(p, precision)
+    s.init_tgv()
+    if model:
+        s.set_les(model, cs=cs, beta=beta, mlp=mlp, gscale=G, delta0=delta0)
+    s.step(5)
+    X.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    s.step(steps)
+    X.synchronize()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    s.close()
+    return e0.elapsed_time(e1) / steps
+
+
 def experiment(n_dns=256, f=4, re=5000.0, dt_dns=0.002, dt_les=0.008, t_end=12.0, snap_times=None, epochs=30,
-               batch=8192, lr=2e-3, cs=0.17, beta=1.0, seed=0, log=None, extra_betas=()):
+               batch=8192, lr=2e-3, cs=0.17, beta=1.0, seed=0, log=None, extra_betas=(), les_precision=0,
+               time_steps=0):
     """The whole learned-closure experiment: filtered DNS, training, and the
     coarse runs without a model, with Smagorinsky (cs), with the learned
     closure in its eddy-viscosity form (model 3, "learned") and as the full
@@ -230,17 +264,21 @@ def experiment(n_dns=256, f=4, re=5000.0, dt_dns=0.002, dt_les=0.008, t_end=12.0
         snap_times = tuple(float(t) for t in range(3, int(t_end) + 1))
     dns = dns_filtered(n_dns, f, re, dt_dns, t_end, snap_times, log=log)
     nc = n_dns // f
-    X.init(X.make_params(nc, nc, nc, re=re, dt=dt_les))
+    _init(X, X.make_params(nc, nc, nc, re=re, dt=dt_les))
     try:
         x, y, G, delta = training_set(dns, log=log)
         mlp, loss = train(x, y, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
         corr = apriori(mlp, x, y)
         runs = {}
-        runs["none"] = run_les(nc, re, dt_les, t_end, model=0)
-        runs["smagorinsky"] = run_les(nc, re, dt_les, t_end, model=1, cs=cs)
-        runs["learned"] = run_les(nc, re, dt_les, t_end, model=3, cs=cs, mlp=mlp, G=G, delta0=delta, beta=beta)
+        pr = les_precision
+        runs["none"] = run_les(nc, re, dt_les, t_end, model=0, precision=pr)
+        runs["smagorinsky"] = run_les(nc, re, dt_les, t_end, model=1, cs=cs, precision=pr)
+        runs["learned"] = run_les(nc, re, dt_les, t_end, model=3, cs=cs, mlp=mlp, G=G, delta0=delta, beta=beta,
+                                  precision=pr)
         runs["learned_full"] = run_les(nc, re, dt_les, t_end, model=2, cs=cs, mlp=mlp, G=G, delta0=delta,
-                                       beta=beta)
+                                       beta=beta, precision=pr)
+        les_ms = (time_les_steps(nc, re, dt_les, 3, mlp=mlp, G=G, delta0=delta, cs=cs, beta=beta, precision=pr,
+                                 steps=time_steps) if time_steps else None)
         for b in extra_betas:
             runs[f"learned_b{b:g}"] = run_les(nc, re, dt_les, t_end, model=3, cs=cs, mlp=mlp, G=G, delta0=delta,
                                               beta=b)
@@ -255,7 +293,8 @@ def experiment(n_dns=256, f=4, re=5000.0, dt_dns=0.002, dt_les=0.008, t_end=12.0
     summary = {
         "dns": {"grid": n_dns, "re": re, "dt": dt_dns, "t_end": t_end, "filter": f,
                 "peak_eps_filtered": [tp, ep], "peak_eps_dns": [tpd, epd]},
-        "les": {"grid": nc, "dt": dt_les, "cs": cs, "beta": beta},
+        "les": {"grid": nc, "dt": dt_les, "cs": cs, "beta": beta, "precision": "fp32" if les_precision else "fp64",
+                "ms_per_step_learned": les_ms},
         "training": {"points": int(x.shape[0]), "G": G, "delta": delta, "final_batch_loss": loss,
                      "apriori_corr": corr, "epochs": epochs},
         "runs": {},
