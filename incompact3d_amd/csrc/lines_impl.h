@@ -218,98 +218,189 @@ __device__ T boundary_row(int i, int n_in, int n_out, const Sten<T> &s, int bc1,
 // elimination factors, and K is chosen on the host so that the product over
 // any K consecutive segments is below the precision's resolution (the terms
 // dropped are smaller than the rounding of the full chain).
-template <typename T, int L, int NR>
-__device__ __forceinline__ void seg_solve(T (&v)[NR][L], const int e0, const T *__restrict__ cs, const T alfa,
-                                          const T den, T *cf, T *cb, T *gf, T *gb, T *x0s, T *xns, const int line,
+//
+// N right-hand sides: the first N1 belong to matrix 1 (table c1, periodic
+// closure alfa1 and 1/den1), the rest to matrix 2.  Every phase runs on all
+// of them between the same three barriers, so the operators of one sweep
+// direction share one set of barriers.  A segment's carry-in reads the
+// carries of the K segments before it (after it, backward) four at a time,
+// the LDS reads ahead of the multiply-add chain.
+//
+// LDS: cf and cb hold N*P*TL carries each, gf and gb 2*P segment factors,
+// x0s and xns N*TL line ends.
+template <typename T, int L, int N, int N1 = N>
+__device__ __forceinline__ void seg_solve(T (&v)[N][L], const int e0, const T *__restrict__ c1, const T alfa1,
+                                          const T rden1, const T *__restrict__ c2, const T alfa2, const T rden2,
+                                          T *cf, T *cb, T *gf, T *gb, T *x0s, T *xns, const int line,
                                           const int seg, const int TL, const int P, const int K) {
-  // NR right-hand sides share the matrix: each coefficient is read once.
+  static_assert(N1 >= 1 && N1 <= N, "matrix 1 needs a right-hand side");
+  constexpr bool TWO = N1 < N;
   const int PT = P * TL;
   // forward elimination, zero carry-in
-  T prev[NR], g = T(1);
+  T prev[N], g1 = T(1), g2 = T(1);
 #pragma unroll
-  for (int q = 0; q < NR; ++q) prev[q] = T(0);
+  for (int q = 0; q < N; ++q) prev[q] = T(0);
 #pragma unroll
   for (int m = 0; m < L; ++m) {
-    const T f = cs[4 * (e0 + m)];
+    const T f1 = c1[4 * (e0 + m)];
+    const T f2 = TWO ? c2[4 * (e0 + m)] : T(0);
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      prev[q] = v[q][m] - prev[q] * f;
+    for (int q = 0; q < N; ++q) {
+      prev[q] = v[q][m] - prev[q] * (q < N1 ? f1 : f2);
       v[q][m] = prev[q];
     }
-    g = -f * g;
+    g1 = -f1 * g1;
+    if (TWO) g2 = -f2 * g2;
   }
 #pragma unroll
-  for (int q = 0; q < NR; ++q) cf[q * PT + seg * TL + line] = prev[q];
-  if (line == 0) gf[seg] = g;
+  for (int q = 0; q < N; ++q) cf[q * PT + seg * TL + line] = prev[q];
+  if (line == 0) {
+    gf[seg] = g1;
+    if (TWO) gf[P + seg] = g2;
+  }
   __syncthreads();
   if (seg > 0) {
-    T cin[NR];
+    T cin[N];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) cin[q] = T(0);
-    for (int s = seg - K > 0 ? seg - K : 0; s < seg; ++s) {
-      const T gs = gf[s];
+    for (int q = 0; q < N; ++q) cin[q] = T(0);
+    for (int s0 = seg - K > 0 ? seg - K : 0; s0 < seg; s0 += 4) {
+      T gs1[4], gs2[4], cv[4][N];
 #pragma unroll
-      for (int q = 0; q < NR; ++q) cin[q] = cf[q * PT + s * TL + line] + gs * cin[q];
+      for (int j = 0; j < 4; ++j) {
+        const int s = s0 + j < seg ? s0 + j : seg - 1;
+        gs1[j] = gf[s];
+        gs2[j] = TWO ? gf[P + s] : T(0);
+#pragma unroll
+        for (int q = 0; q < N; ++q) cv[j][q] = cf[q * PT + s * TL + line];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (s0 + j < seg) {
+#pragma unroll
+          for (int q = 0; q < N; ++q) cin[q] = cv[j][q] + (q < N1 ? gs1[j] : gs2[j]) * cin[q];
+        }
+      }
     }
-    T gg = T(1);
+    T gg1 = T(1), gg2 = T(1);
 #pragma unroll
     for (int m = 0; m < L; ++m) {
-      gg = -cs[4 * (e0 + m)] * gg;
+      gg1 = -c1[4 * (e0 + m)] * gg1;
+      if (TWO) gg2 = -c2[4 * (e0 + m)] * gg2;
 #pragma unroll
-      for (int q = 0; q < NR; ++q) v[q][m] += gg * cin[q];
+      for (int q = 0; q < N; ++q) v[q][m] += (q < N1 ? gg1 : gg2) * cin[q];
     }
   }
   // back substitution, zero carry-in
-  T nxt[NR], h = T(1);
+  T nxt[N], h1 = T(1), h2 = T(1);
 #pragma unroll
-  for (int q = 0; q < NR; ++q) nxt[q] = T(0);
+  for (int q = 0; q < N; ++q) nxt[q] = T(0);
 #pragma unroll
   for (int m = L - 1; m >= 0; --m) {
-    const T w = cs[4 * (e0 + m) + 1], ffw = cs[4 * (e0 + m) + 2];
+    const T w1 = c1[4 * (e0 + m) + 1], ffw1 = c1[4 * (e0 + m) + 2];
+    const T w2 = TWO ? c2[4 * (e0 + m) + 1] : T(0), ffw2 = TWO ? c2[4 * (e0 + m) + 2] : T(0);
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-      nxt[q] = w * v[q][m] - ffw * nxt[q];
+    for (int q = 0; q < N; ++q) {
+      nxt[q] = (q < N1 ? w1 : w2) * v[q][m] - (q < N1 ? ffw1 : ffw2) * nxt[q];
       v[q][m] = nxt[q];
     }
-    h = -ffw * h;
+    h1 = -ffw1 * h1;
+    if (TWO) h2 = -ffw2 * h2;
   }
 #pragma unroll
-  for (int q = 0; q < NR; ++q) cb[q * PT + seg * TL + line] = nxt[q];
-  if (line == 0) gb[seg] = h;
+  for (int q = 0; q < N; ++q) cb[q * PT + seg * TL + line] = nxt[q];
+  if (line == 0) {
+    gb[seg] = h1;
+    if (TWO) gb[P + seg] = h2;
+  }
   __syncthreads();
   if (seg < P - 1) {
-    T c[NR];
+    T c[N];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) c[q] = T(0);
-    for (int s = seg + K < P - 1 ? seg + K : P - 1; s > seg; --s) {
-      const T gs = gb[s];
+    for (int q = 0; q < N; ++q) c[q] = T(0);
+    const int last = seg + K < P - 1 ? seg + K : P - 1;
+    for (int s0 = last; s0 > seg; s0 -= 4) {
+      T gs1[4], gs2[4], cv[4][N];
 #pragma unroll
-      for (int q = 0; q < NR; ++q) c[q] = cb[q * PT + s * TL + line] + gs * c[q];
+      for (int j = 0; j < 4; ++j) {
+        const int s = s0 - j > seg ? s0 - j : seg + 1;
+        gs1[j] = gb[s];
+        gs2[j] = TWO ? gb[P + s] : T(0);
+#pragma unroll
+        for (int q = 0; q < N; ++q) cv[j][q] = cb[q * PT + s * TL + line];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (s0 - j > seg) {
+#pragma unroll
+          for (int q = 0; q < N; ++q) c[q] = cv[j][q] + (q < N1 ? gs1[j] : gs2[j]) * c[q];
+        }
+      }
     }
-    T hh = T(1);
+    T hh1 = T(1), hh2 = T(1);
 #pragma unroll
     for (int m = L - 1; m >= 0; --m) {
-      hh = -cs[4 * (e0 + m) + 2] * hh;
+      hh1 = -c1[4 * (e0 + m) + 2] * hh1;
+      if (TWO) hh2 = -c2[4 * (e0 + m) + 2] * hh2;
 #pragma unroll
-      for (int q = 0; q < NR; ++q) v[q][m] += hh * c[q];
+      for (int q = 0; q < N; ++q) v[q][m] += (q < N1 ? hh1 : hh2) * c[q];
     }
   }
   // periodic closure (Sherman-Morrison with the precomputed vector sm)
 #pragma unroll
-  for (int q = 0; q < NR; ++q) {
+  for (int q = 0; q < N; ++q) {
     if (seg == 0) x0s[q * TL + line] = v[q][0];
     if (seg == P - 1) xns[q * TL + line] = v[q][L - 1];
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    const T sx = (x0s[q * TL + line] - alfa * xns[q * TL + line]) / den;
+  for (int q = 0; q < N; ++q) {
+    const T sx = (x0s[q * TL + line] - (q < N1 ? alfa1 : alfa2) * xns[q * TL + line]) * (q < N1 ? rden1 : rden2);
+    const T *cq = q < N1 ? c1 : c2;
 #pragma unroll
-    for (int m = 0; m < L; ++m) v[q][m] = v[q][m] - sx * cs[4 * (e0 + m) + 3];
+    for (int m = 0; m < L; ++m) v[q][m] = v[q][m] - sx * cq[4 * (e0 + m) + 3];
   }
   // No barrier needed before the next solve: it writes cf/gf before its
   // first barrier (last read here before the second), cb/gb after its first
   // barrier (last read here before the third) and x0s/xns after its second.
+}
+
+// Copy of two coefficient tables (cnt words each) into LDS split around the
+// caller's window loads: load() issues the first R*nthr words of each into
+// registers, store() writes them and copies any rest, so the table and window
+// latencies overlap before the barrier that must follow store().
+template <typename T, int R>
+struct TableCopy {
+  T v1[R], v2[R];
+  __device__ __forceinline__ void load(const T *a, const T *b, int tid, int nthr, int cnt) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = tid + r * nthr;
+      if (i < cnt) {
+        v1[r] = a[i];
+        v2[r] = b[i];
+      }
+    }
+  }
+  __device__ __forceinline__ void store(T *da, T *db, const T *a, const T *b, int tid, int nthr, int cnt) const {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int i = tid + r * nthr;
+      if (i < cnt) {
+        da[i] = v1[r];
+        db[i] = v2[r];
+      }
+    }
+    for (int i = tid + R * nthr; i < cnt; i += nthr) {
+      da[i] = a[i];
+      db[i] = b[i];
+    }
+  }
+};
+
+// LDS words of a segmented sweep: nmat coefficient tables, the carries and
+// line ends of nrhs right-hand sides (the layout of seg_solve).
+inline size_t seg_lds_words(int n, int P, int TL, int nmat, int nrhs) {
+  return 4 * (size_t)nmat * n + 2 * (size_t)nrhs * P * TL + 2 * (size_t)nmat * P + 2 * (size_t)nrhs * TL;
 }
 
 template <typename T>
@@ -405,27 +496,22 @@ __device__ __forceinline__ void wave_solve(T (&v)[V], const T (&fs)[V], const T 
 // Launch shape of a segmented y/z sweep for a line of n points: segments of
 // L = 8 (FP64) or 16 (FP32) points, so that the fused sweeps' three solves
 // keep their windows, right-hand sides and results within 128 VGPRs; TL
-// lines per block with TL*P <= 1024 threads and the LDS within 64 KiB --
-// for the fused sweeps two coefficient tables and the carries of two
-// right-hand sides, for a single operator (single = true) one of each.
+// lines per block with TL*P <= 1024 threads and the LDS (seg_lds_words)
+// within 64 KiB.
 template <typename T>
 struct YzShape {
   static constexpr int L = sizeof(T) == 8 ? 8 : 16;
   int P = 0, TL = 0;
   size_t shmem = 0;
-  static size_t lds(int n, int P, int TL, bool single) {
-    const size_t k = single ? 1 : 2;
-    return sizeof(T) * (4 * k * (size_t)n + 2 * k * (size_t)P * TL + 2 * (size_t)P + 2 * k * (size_t)TL);
-  }
-  bool plan(int n, bool single = false) {
+  bool plan(int n, int nmat, int nrhs) {
     if (n < 8 || n % L != 0) return false;
     P = n / L;
     // 128-byte rows (16 FP64 / 32 FP32 lines): full cache lines per point,
     // and at 256-point lines 512-thread blocks, two of which share a CU
     TL = 128 / (int)sizeof(T);
-    while (TL >= 8 && (TL * P > 1024 || lds(n, P, TL, single) > 65536)) TL >>= 1;
+    while (TL >= 8 && (TL * P > 1024 || sizeof(T) * seg_lds_words(n, P, TL, nmat, nrhs) > 65536)) TL >>= 1;
     if (TL < 8) return false;
-    shmem = lds(n, P, TL, single);
+    shmem = sizeof(T) * seg_lds_words(n, P, TL, nmat, nrhs);
     return true;
   }
 };
@@ -663,7 +749,7 @@ __global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
 // the epilogue's inputs read before the first store (acc may be t).  Byte
 // offsets are 32-bit (the host checks the field size).
 template <typename T, int OP, int L, bool EPI>
-__global__ void __launch_bounds__(1024) seg_line_kernel(const LineArgs<T> A, const int kdepth) {
+__global__ void __launch_bounds__(1024) seg_line_kernel(const LineArgs<T> A, const T rden, const int kdepth) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
   const int TL = blockDim.x, P = blockDim.y;
@@ -672,29 +758,30 @@ __global__ void __launch_bounds__(1024) seg_line_kernel(const LineArgs<T> A, con
   const int n = A.n_out;
   T *cs = smem, *cf = cs + 4 * n, *cb = cf + P * TL, *gf = cb + P * TL, *gb = gf + P, *x0s = gb + P,
     *xns = x0s + TL;
-  for (int i = tid; i < 4 * n; i += nthr) cs[i] = A.pk[i];
-  __syncthreads();
   const int a = blockIdx.x * TL + line, b = blockIdx.y;
   const bool active = a < A.na;
+  // lines past the end compute on a copy of the last line (no load branches)
+  const int al = active ? a : A.na - 1;
   const int e0 = seg * L;
-  const unsigned base = ((unsigned)a * (unsigned)A.sa_in + (unsigned)b * (unsigned)A.sb_in) * (unsigned)sizeof(T);
+  const unsigned base = ((unsigned)al * (unsigned)A.sa_in + (unsigned)b * (unsigned)A.sb_in) * (unsigned)sizeof(T);
   const unsigned step = (unsigned)A.se_in * (unsigned)sizeof(T);
   auto wrap = [&](int j) { return (unsigned)(j < 0 ? j + n : (j >= n ? j - n : j)); };
+  // the window loads go out before the coefficient copy and its barrier
   T win[L + 8];
 #pragma unroll
   for (int k = 0; k < L + 8; ++k) {
-    T x = T(0);
-    if (active) {
-      const unsigned o = base + wrap(e0 - 4 + k) * step;
-      x = at(A.u, o);
-      if (EPI && A.u2) x *= at(A.u2, o);
-    }
+    const unsigned o = base + wrap(e0 - 4 + k) * step;
+    T x = at(A.u, o);
+    if (EPI && A.u2) x *= at(A.u2, o);
     win[k] = x;
   }
+  for (int i = tid; i < 4 * n; i += nthr) cs[i] = A.pk[i];
+  __syncthreads();
   T w[1][L];
 #pragma unroll
   for (int m = 0; m < L; ++m) w[0][m] = interior_row<OP, T, L + 8>(win, m + 4, A.s);
-  seg_solve<T, L, 1>(w, e0, cs, A.alfa, A.sm_den, cf, cb, gf, gb, x0s, xns, line, seg, TL, P, kdepth);
+  seg_solve<T, L, 1>(w, e0, cs, A.alfa, rden, cs, A.alfa, rden, cf, cb, gf, gb, x0s, xns, line, seg, TL, P,
+                     kdepth);
   if (!active) return;
   if (A.pp) {
 #pragma unroll
@@ -723,13 +810,14 @@ __global__ void __launch_bounds__(1024) seg_line_kernel(const LineArgs<T> A, con
 }
 
 template <typename T, int OP>
-static int launch_seg_line(const LineArgs<T> &a, const YzShape<T> &Y, bool epi, int kdepth, hipStream_t st) {
+static int launch_seg_line(const LineArgs<T> &a, T rden, const YzShape<T> &Y, bool epi, int kdepth,
+                           hipStream_t st) {
   // the segment solve needs one line per block row and every segment full
   const dim3 grid((a.na + Y.TL - 1) / Y.TL, a.nb), block(Y.TL, Y.P);
   const size_t shmem = Y.shmem;
   constexpr int L = YzShape<T>::L;
-  if (epi) hipLaunchKernelGGL((seg_line_kernel<T, OP, L, true>), grid, block, shmem, st, a, kdepth);
-  else hipLaunchKernelGGL((seg_line_kernel<T, OP, L, false>), grid, block, shmem, st, a, kdepth);
+  if (epi) hipLaunchKernelGGL((seg_line_kernel<T, OP, L, true>), grid, block, shmem, st, a, rden, kdepth);
+  else hipLaunchKernelGGL((seg_line_kernel<T, OP, L, false>), grid, block, shmem, st, a, rden, kdepth);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("segmented line launch: ") + hipGetErrorString(e));
   return X3D_OK;
@@ -1042,15 +1130,16 @@ int launch_line(const LineCall<T> &c, hipStream_t st) {
     }
   }
   YzShape<T> Y;
-  if (!xmode && periodic && Y.plan(n_out, true) && (size_t)nx * ny * nz * sizeof(T) < ((size_t)1 << 32)) {
+  if (!xmode && periodic && Y.plan(n_out, 1, 1) && (size_t)nx * ny * nz * sizeof(T) < ((size_t)1 << 32)) {
     const int K = c.tri.kdepth > 0 ? c.tri.kdepth : Y.P;
+    const T rden = c.tri.rden;
     switch (c.op) {
-      case OP_DER1: return launch_seg_line<T, OP_DER1>(a, Y, epi, K, st);
-      case OP_DER2: return launch_seg_line<T, OP_DER2>(a, Y, epi, K, st);
-      case OP_DERVP: return launch_seg_line<T, OP_DERVP>(a, Y, epi, K, st);
-      case OP_INTERVP: return launch_seg_line<T, OP_INTERVP>(a, Y, epi, K, st);
-      case OP_DERPV: return launch_seg_line<T, OP_DERPV>(a, Y, epi, K, st);
-      case OP_INTERPV: return launch_seg_line<T, OP_INTERPV>(a, Y, epi, K, st);
+      case OP_DER1: return launch_seg_line<T, OP_DER1>(a, rden, Y, epi, K, st);
+      case OP_DER2: return launch_seg_line<T, OP_DER2>(a, rden, Y, epi, K, st);
+      case OP_DERVP: return launch_seg_line<T, OP_DERVP>(a, rden, Y, epi, K, st);
+      case OP_INTERVP: return launch_seg_line<T, OP_INTERVP>(a, rden, Y, epi, K, st);
+      case OP_DERPV: return launch_seg_line<T, OP_DERPV>(a, rden, Y, epi, K, st);
+      case OP_INTERPV: return launch_seg_line<T, OP_INTERPV>(a, rden, Y, epi, K, st);
     }
   }
   const int ax = c.dir;

@@ -8,8 +8,9 @@
 // and accumulates  -1/2 D1_d(u_d u_c) - 1/2 u_d D1_d(u_c) + nu D2_d(u_c)
 // into the running right-hand side, so each (d, c) pair moves four fields
 // through HBM (u_d, u_c, the accumulator in, the accumulator out) instead of
-// ten.  Each input window is read once; the two first-derivative systems
-// share their matrix and are solved together.  The x sweep, the last one in the
+// ten.  Each input window is read once; the three solves (two on the
+// first-derivative matrix, one on the second-derivative matrix) run between
+// one set of block barriers.  The x sweep, the last one in the
 // reference's order, also applies AB3 (time_integrators.f90:85-100): the new
 // velocity goes to a second buffer, because the other components still read
 // the old one.
@@ -23,6 +24,7 @@ struct MomArgs {
   T *out;
   const T *pk1, *pk2;           // packed first/second-derivative sets
   T alfa1, den1, alfa2, den2;
+  T rden1, rden2;               // 1 / den
   Sten<T> s;
   T xnu;
   int n, na, nb;
@@ -46,35 +48,36 @@ __global__ void __launch_bounds__(1024) mom_yz_kernel(const MomArgs<T> A) {
   const int line = threadIdx.x, seg = threadIdx.y;
   const int tid = seg * TL + line, nthr = TL * P;
   const int n = A.n;
-  // LDS: two coefficient tables, carries of two right-hand sides
-  T *cs1 = smem, *cs2 = cs1 + 4 * n, *cf = cs2 + 4 * n, *cb = cf + 2 * P * TL;
-  T *gf = cb + 2 * P * TL, *gb = gf + P, *x0s = gb + P, *xns = x0s + 2 * TL;
-  for (int i = tid; i < 4 * n; i += nthr) {
-    cs1[i] = A.pk1[i];
-    cs2[i] = A.pk2[i];
-  }
-  __syncthreads();
+  // LDS: two coefficient tables, carries of three right-hand sides
+  T *cs1 = smem, *cs2 = cs1 + 4 * n, *cf = cs2 + 4 * n, *cb = cf + 3 * P * TL;
+  T *gf = cb + 3 * P * TL, *gb = gf + 2 * P, *x0s = gb + 2 * P, *xns = x0s + 3 * TL;
   const int a = blockIdx.x * TL + line, b = blockIdx.y;
   const bool active = a < A.na;
+  // lines past the end compute on a copy of the last line (no load branches)
+  const int al = active ? a : A.na - 1;
   const int e0 = seg * L;        // n = P*L: every segment is full
   // byte offsets in 32 bits (the host checks the field size), so each load is
   // a uniform base plus one VGPR offset
-  const unsigned base = ((unsigned)a * (unsigned)A.sa + (unsigned)b * (unsigned)A.sb) * (unsigned)sizeof(T);
+  const unsigned base = ((unsigned)al * (unsigned)A.sa + (unsigned)b * (unsigned)A.sb) * (unsigned)sizeof(T);
   const unsigned step = (unsigned)A.se * (unsigned)sizeof(T);
   // windows reach at most 4 points past either end of the line (n >= 8)
   auto wrap = [&](int j) { return (unsigned)(j < 0 ? j + n : (j >= n ? j - n : j)); };
-  // right-hand sides: w[0] of D1(u_d u_c), w[1] of D1(u_c), w2 of D2(u_c);
-  // u_c is read over [e0-4, e0+L+4), u_d over [e0-2, e0+L+2), each once
-  T w[2][L], w2[1][L], ud[L];
+  // right-hand sides: w[0] of D1(u_d u_c), w[1] of D1(u_c) (first-derivative
+  // matrix), w[2] of D2(u_c) (second-derivative matrix); u_c is read over
+  // [e0-4, e0+L+4), u_d over [e0-2, e0+L+2), each once, before the
+  // coefficient copy and its barrier
+  TableCopy<T, 2> tc;
+  tc.load(A.pk1, A.pk2, tid, nthr, 4 * n);
+  T w[3][L], ud[L];
   {
     T wc[L + 8];
 #pragma unroll
-    for (int k = 0; k < L + 8; ++k) wc[k] = active ? at(A.uc, base + wrap(e0 - 4 + k) * step) : T(0);
+    for (int k = 0; k < L + 8; ++k) wc[k] = at(A.uc, base + wrap(e0 - 4 + k) * step);
     {
       T p[L + 4];
 #pragma unroll
       for (int k = 0; k < L + 4; ++k) {
-        const T d = active ? at(A.ud, base + wrap(e0 - 2 + k) * step) : T(0);
+        const T d = at(A.ud, base + wrap(e0 - 2 + k) * step);
         p[k] = d * wc[k + 2];
         if (k >= 2 && k < L + 2) ud[k - 2] = d;
       }
@@ -84,23 +87,28 @@ __global__ void __launch_bounds__(1024) mom_yz_kernel(const MomArgs<T> A) {
 #pragma unroll
     for (int m = 0; m < L; ++m) {
       w[1][m] = interior_row<OP_DER1, T, L + 8>(wc, m + 4, A.s);
-      w2[0][m] = interior_row<OP_DER2, T, L + 8>(wc, m + 4, A.s);
+      w[2][m] = interior_row<OP_DER2, T, L + 8>(wc, m + 4, A.s);
     }
   }
-  seg_solve<T, L, 2>(w, e0, cs1, A.alfa1, A.den1, cf, cb, gf, gb, x0s, xns, line, seg, TL, P, A.kdepth);
-  seg_solve<T, L, 1>(w2, e0, cs2, A.alfa2, A.den2, cf, cb, gf, gb, x0s, xns, line, seg, TL, P, A.kdepth);
-  if (active) {
+  tc.store(cs1, cs2, A.pk1, A.pk2, tid, nthr, 4 * n);
+  __syncthreads();
+  seg_solve<T, L, 3, 2>(w, e0, cs1, A.alfa1, A.rden1, cs2, A.alfa2, A.rden2, cf, cb, gf, gb, x0s, xns, line, seg,
+                        TL, P, A.kdepth);
+  if (!active) return;
+  // -1/2 D1(u_d u_c) - 1/2 u_d D1(u_c) + nu D2(u_c)
+#pragma unroll
+  for (int m = 0; m < L; ++m) w[0][m] = T(-0.5) * w[0][m] + T(-0.5) * (ud[m] * w[1][m]) + A.xnu * w[2][m];
+  if (A.acc) {
     // acc may be out (in-place accumulation): all of this thread's acc
     // points are read before any is written
     T ac[L];
 #pragma unroll
-    for (int m = 0; m < L; ++m) ac[m] = A.acc ? at(A.acc, base + (unsigned)(e0 + m) * step) : T(0);
+    for (int m = 0; m < L; ++m) ac[m] = at(A.acc, base + (unsigned)(e0 + m) * step);
 #pragma unroll
-    for (int m = 0; m < L; ++m) {
-      // -1/2 D1(u_d u_c) - 1/2 u_d D1(u_c) + nu D2(u_c)
-      const T r = T(-0.5) * w[0][m] + T(-0.5) * (ud[m] * w[1][m]) + A.xnu * w2[0][m];
-      at(A.out, base + (unsigned)(e0 + m) * step) = ac[m] + r;
-    }
+    for (int m = 0; m < L; ++m) at(A.out, base + (unsigned)(e0 + m) * step) = ac[m] + w[0][m];
+  } else {
+#pragma unroll
+    for (int m = 0; m < L; ++m) at(A.out, base + (unsigned)(e0 + m) * step) = w[0][m];
   }
 }
 
@@ -209,7 +217,7 @@ bool momentum_fusable(const Context &c) {
     if (d == 0) {
       const int V = D.n / 64;
       if (D.n % 64 != 0 || !(V == 1 || V == 2 || V == 4 || V == 8)) return false;
-    } else if (!YzShape<double>().plan(D.n) || !YzShape<float>().plan(D.n)) {
+    } else if (!YzShape<double>().plan(D.n, 2, 3) || !YzShape<float>().plan(D.n, 2, 3)) {
       return false;
     }
   }
@@ -237,6 +245,7 @@ int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], co
   else { t1 = s1.dev32(); t2 = s2.dev32(); }
   A.pk1 = t1.pk; A.pk2 = t2.pk;
   A.alfa1 = t1.alfa; A.den1 = t1.sm_den; A.alfa2 = t2.alfa; A.den2 = t2.sm_den;
+  A.rden1 = t1.rden; A.rden2 = t2.rden;
   A.s = cast_sten<T>(D.s);
   A.xnu = (T)c.xnu;
   A.n = D.n;
@@ -265,11 +274,10 @@ int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], co
     }
     constexpr int L = YzShape<T>::L;
     YzShape<T> Y;
-    if (!Y.plan(n)) return fail(X3D_ERR_UNSUPPORTED, "fused y/z sweep: line length not supported");
+    if (!Y.plan(n, 2, 3)) return fail(X3D_ERR_UNSUPPORTED, "fused y/z sweep: line length not supported");
     if ((size_t)nx * ny * nz * sizeof(T) >= ((size_t)1 << 32))
       return fail(X3D_ERR_UNSUPPORTED, "fused y/z sweep: field larger than 4 GiB");
-    const double tol = sizeof(T) == 8 ? 1e-18 : 1e-9;
-    A.kdepth = std::max(chain_depth(s1, L, tol), chain_depth(s2, L, tol));
+    A.kdepth = std::max(t1.kdepth, t2.kdepth);
     const dim3 grid((A.na + Y.TL - 1) / Y.TL, A.nb), block(Y.TL, Y.P);
     const size_t shmem = Y.shmem;
     hipLaunchKernelGGL((mom_yz_kernel<T, L>), grid, block, shmem, st, A);

@@ -64,6 +64,7 @@ struct ProjArgs {
   T *out[3];
   const T *pkI, *pkD;           // packed interpolation and derivative sets
   T alfaI, denI, alfaD, denD;
+  T rdenI, rdenD;               // 1 / den
   Sten<T> s;
   int n, na, nb;
   long long sa, sb, se;         // line strides (y/z sweeps)
@@ -84,59 +85,62 @@ __global__ void __launch_bounds__(1024) proj_yz_kernel(const ProjArgs<T> A) {
   const int line = threadIdx.x, seg = threadIdx.y;
   const int tid = seg * TL + line, nthr = TL * P;
   const int n = A.n;
-  T *csI = smem, *csD = csI + 4 * n, *cf = csD + 4 * n, *cb = cf + 2 * P * TL;
-  T *gf = cb + 2 * P * TL, *gb = gf + P, *x0s = gb + P, *xns = x0s + 2 * TL;
-  for (int i = tid; i < 4 * n; i += nthr) {
-    csI[i] = A.pkI[i];
-    csD[i] = A.pkD[i];
-  }
-  __syncthreads();
+  constexpr int N = R::NI + 1;   // interpolations, then the derivative
+  T *csI = smem, *csD = csI + 4 * n, *cf = csD + 4 * n, *cb = cf + N * P * TL;
+  T *gf = cb + N * P * TL, *gb = gf + 2 * P, *x0s = gb + 2 * P, *xns = x0s + N * TL;
   const int a = blockIdx.x * TL + line, b = blockIdx.y;
   const bool active = a < A.na;
+  // lines past the end compute on a copy of the last line (no load branches)
+  const int al = active ? a : A.na - 1;
   const int e0 = seg * L;        // n = P*L: every segment is full
-  const unsigned base = ((unsigned)a * (unsigned)A.sa + (unsigned)b * (unsigned)A.sb) * (unsigned)sizeof(T);
+  const unsigned base = ((unsigned)al * (unsigned)A.sa + (unsigned)b * (unsigned)A.sb) * (unsigned)sizeof(T);
   const unsigned step = (unsigned)A.se * (unsigned)sizeof(T);
   auto wrap = [&](int j) { return (unsigned)(j < 0 ? j + n : (j >= n ? j - n : j)); };
-  T wI[R::NI][L], wD[1][L];
+  TableCopy<T, 2> tc;
+  tc.load(A.pkI, A.pkD, tid, nthr, 4 * n);
+  // w[0..NI-1]: interpolations, w[NI]: derivative
+  T w[N][L];
 #pragma unroll
   for (int i = 0; i < R::NIN; ++i) {
     // window U(e0-4 .. e0+L+3) of input i; stencils reach at most 4 points
     T win[L + 8];
 #pragma unroll
-    for (int k = 0; k < L + 8; ++k) win[k] = active ? at(A.in[i], base + wrap(e0 - 4 + k) * step) : T(0);
+    for (int k = 0; k < L + 8; ++k) win[k] = at(A.in[i], base + wrap(e0 - 4 + k) * step);
     if (i == R::I0) {
 #pragma unroll
-      for (int m = 0; m < L; ++m) wI[0][m] = interior_row<OPI, T, L + 8>(win, m + 4, A.s);
+      for (int m = 0; m < L; ++m) w[0][m] = interior_row<OPI, T, L + 8>(win, m + 4, A.s);
     }
     if constexpr (R::NI == 2) {
       if (i == R::I1) {
 #pragma unroll
-        for (int m = 0; m < L; ++m) wI[1][m] = interior_row<OPI, T, L + 8>(win, m + 4, A.s);
+        for (int m = 0; m < L; ++m) w[1][m] = interior_row<OPI, T, L + 8>(win, m + 4, A.s);
       }
     }
     if (i == R::D0) {
 #pragma unroll
-      for (int m = 0; m < L; ++m) wD[0][m] = interior_row<OPD, T, L + 8>(win, m + 4, A.s);
+      for (int m = 0; m < L; ++m) w[N - 1][m] = interior_row<OPD, T, L + 8>(win, m + 4, A.s);
     }
   }
-  seg_solve<T, L, R::NI>(wI, e0, csI, A.alfaI, A.denI, cf, cb, gf, gb, x0s, xns, line, seg, TL, P, A.kdepth);
-  seg_solve<T, L, 1>(wD, e0, csD, A.alfaD, A.denD, cf, cb, gf, gb, x0s, xns, line, seg, TL, P, A.kdepth);
+  tc.store(csI, csD, A.pkI, A.pkD, tid, nthr, 4 * n);
+  __syncthreads();
+  seg_solve<T, L, N, R::NI>(w, e0, csI, A.alfaI, A.rdenI, csD, A.alfaD, A.rdenD, cf, cb, gf, gb, x0s, xns, line,
+                            seg, TL, P, A.kdepth);
   if (active) {
 #pragma unroll
     for (int m = 0; m < L; ++m) {
       const unsigned o = base + (unsigned)(e0 + m) * step;
       if constexpr (SW == SW_DIV_Y) {
-        at(A.out[0], o) = wI[0][m] + wD[0][m];
-        at(A.out[1], o) = wI[1][m];
+        at(A.out[0], o) = w[0][m] + w[N - 1][m];
+        at(A.out[1], o) = w[1][m];
       } else if constexpr (SW == SW_DIV_Z) {
-        at(A.out[0], o) = wI[0][m] + wD[0][m];
+        at(A.out[0], o) = w[0][m] + w[N - 1][m];
       } else if constexpr (SW == SW_GRAD_Z) {
-        at(A.out[0], o) = wI[0][m];
-        at(A.out[1], o) = wD[0][m];
+        at(A.out[0], o) = w[0][m];
+        at(A.out[1], o) = w[N - 1][m];
       } else {   // SW_GRAD_Y
-        at(A.out[0], o) = wI[0][m];
-        at(A.out[1], o) = wD[0][m];
-        at(A.out[2], o) = wI[1][m];
+        at(A.out[0], o) = w[0][m];
+        at(A.out[1], o) = w[N - 1][m];
+        at(A.out[2], o) = w[1][m];
       }
     }
   }
@@ -224,7 +228,7 @@ bool projection_fusable(const Context &c) {
     if (d == 0) {
       const int V = D.n / 64;
       if (D.n % 64 != 0 || !(V == 1 || V == 2 || V == 4 || V == 8)) return false;
-    } else if (!YzShape<double>().plan(D.n) || !YzShape<float>().plan(D.n)) {
+    } else if (!YzShape<double>().plan(D.n, 2, 3) || !YzShape<float>().plan(D.n, 2, 3)) {
       return false;
     }
   }
@@ -273,6 +277,7 @@ int launch_projection(const Context &c, int sw, const T *const in[3], T *const o
   else { tI = sI.dev32(); tD = sD.dev32(); }
   A.pkI = tI.pk; A.pkD = tD.pk;
   A.alfaI = tI.alfa; A.denI = tI.sm_den; A.alfaD = tD.alfa; A.denD = tD.sm_den;
+  A.rdenI = tI.rden; A.rdenD = tD.rden;
   A.s = cast_sten<T>(D.s);
   A.n = D.n;
   const int n = D.n;
@@ -292,11 +297,10 @@ int launch_projection(const Context &c, int sw, const T *const in[3], T *const o
       A.na = nx; A.nb = ny; A.sa = 1; A.sb = nx; A.se = (long long)nx * ny;
     }
     YzShape<T> Y;
-    if (!Y.plan(n)) return fail(X3D_ERR_UNSUPPORTED, "fused y/z sweep: line length not supported");
+    if (!Y.plan(n, 2, 3)) return fail(X3D_ERR_UNSUPPORTED, "fused y/z sweep: line length not supported");
     if ((size_t)nx * ny * nz * sizeof(T) >= ((size_t)1 << 32))
       return fail(X3D_ERR_UNSUPPORTED, "fused y/z sweep: field larger than 4 GiB");
-    const double tol = sizeof(T) == 8 ? 1e-18 : 1e-9;
-    A.kdepth = std::max(chain_depth(sI, YzShape<T>::L, tol), chain_depth(sD, YzShape<T>::L, tol));
+    A.kdepth = std::max(tI.kdepth, tD.kdepth);
     switch (sw) {
       case SW_DIV_Y: launch_yz<T, SW_DIV_Y>(A, Y, st); break;
       case SW_DIV_Z: launch_yz<T, SW_DIV_Z>(A, Y, st); break;

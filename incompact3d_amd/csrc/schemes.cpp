@@ -267,14 +267,16 @@ void cast_to(const Sten<double> &a, Sten<T> &b) {
 TriDev<double> TriSet::dev64() const {
   TriDev<double> d;
   d.pk = d64;
-  d.alfa = alfa; d.sm_den = sm_den; d.n = (int)ff.size(); d.periodic = periodic; d.kdepth = kd8;
+  d.alfa = alfa; d.sm_den = sm_den; d.rden = 1.0 / sm_den;
+  d.n = (int)ff.size(); d.periodic = periodic; d.kdepth = kd8;
   return d;
 }
 
 TriDev<float> TriSet::dev32() const {
   TriDev<float> d;
   d.pk = d32;
-  d.alfa = (float)alfa; d.sm_den = (float)sm_den; d.n = (int)ff.size(); d.periodic = periodic; d.kdepth = kd16;
+  d.alfa = (float)alfa; d.sm_den = (float)sm_den; d.rden = (float)(1.0 / sm_den);
+  d.n = (int)ff.size(); d.periodic = periodic; d.kdepth = kd16;
   return d;
 }
 

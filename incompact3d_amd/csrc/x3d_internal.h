@@ -53,7 +53,7 @@ struct Sten {
 template <typename T>
 struct TriDev {
   const T *pk = nullptr;
-  T alfa = 0, sm_den = 1;
+  T alfa = 0, sm_den = 1, rden = 1;   // rden = 1 / sm_den
   int n = 0;
   int periodic = 0;
   int kdepth = 0;               // segment carry-chain depth (0: full chain)
