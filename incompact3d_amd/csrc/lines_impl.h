@@ -40,7 +40,7 @@ struct LineArgs {
   const T *acc, *mul;                 // epilogue: t = ac*acc + rc*(mul*result)
   T ac, rc;
   const T *pk;                        // packed {fs, fw, ff*fw, r} per point
-  T alfa, sm_den;
+  T alfa, sm_den, rden;                // rden = 1 / sm_den
   Sten<T> s;
   int bc1, bcn, npaire;
   int n_in, n_out;
@@ -416,6 +416,31 @@ __device__ __forceinline__ const T &at(const T *p, unsigned byte_off) {
 // floor(a / b) for b > 0 (compile-time after unrolling)
 constexpr int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
+// ---------------------------------------------------------------- cross-lane moves
+// DPP moves (VALU, no LDS round trip) of 32- and 64-bit values.  CTRL is the
+// DPP control: 0x101-0x10F row_shl (lane l reads lane l+k of its row of 16),
+// 0x111-0x11F row_shr (lane l reads l-k), 0x130 wave_shl:1, 0x134 wave_rol:1,
+// 0x138 wave_shr:1, 0x13C wave_ror:1.  Lanes without a source keep `old`.
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x, float old) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double x, double old) {
+  const long long xi = __double_as_longlong(x), oi = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)oi, (int)xi, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(oi >> 32), (int)(xi >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float readlane(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double readlane(double x, int l) {
+  const long long xi = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)xi, l), hi = __builtin_amdgcn_readlane((int)(xi >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 template <typename T, int V, int HL, int HR>
 __device__ __forceinline__ void halo_window(const T (&own)[V], T (&v)[V + HL + HR], const int lane) {
 #pragma unroll
@@ -426,14 +451,70 @@ __device__ __forceinline__ void halo_window(const T (&own)[V], T (&v)[V + HL + H
     } else {
       const int lo = floor_div(d, V);
       const int el = d - lo * V;
-      v[k] = __shfl(own[el], (lane + lo) & 63);
+      if (lo == -1) v[k] = dpp<0x13C>(own[el], T(0));        // wave_ror:1, lane l reads l-1 (0 reads 63)
+      else if (lo == 1) v[k] = dpp<0x134>(own[el], T(0));    // wave_rol:1, lane l reads l+1 (63 reads 0)
+      else v[k] = __shfl(own[el], (lane + lo) & 63);
     }
   }
 }
 
+// Composition of the affine maps x -> a x + b of the 64 lanes of a
+// wavefront.  scan_up leaves in lane l the maps of lanes 0..l applied in lane
+// order, scan_down the maps of lanes l..63 applied from lane 63 down: four
+// DPP steps within each row of 16 lanes, then the row totals (read with
+// readlane) carried across the four rows.
+template <typename T>
+__device__ __forceinline__ void scan_up(T &a, T &b, const int lane) {
+#define X3D_UP(S)                                                                                                      \
+  {                                                                                                                    \
+    const T ap = dpp<0x110 + S>(a, T(1)), bp = dpp<0x110 + S>(b, T(0));                                               \
+    b = a * bp + b;                                                                                                    \
+    a = a * ap;                                                                                                        \
+  }
+  X3D_UP(1) X3D_UP(2) X3D_UP(4) X3D_UP(8)
+#undef X3D_UP
+  const T a0 = readlane(a, 15), b0 = readlane(b, 15);
+  const T a1 = readlane(a, 31), b1 = readlane(b, 31);
+  const T a2 = readlane(a, 47), b2 = readlane(b, 47);
+  // carry into rows 1, 2, 3: rows 0..r-1 composed
+  const T c2a = a1 * a0, c2b = a1 * b0 + b1;
+  const T c3a = a2 * c2a, c3b = a2 * c2b + b2;
+  const int row = lane >> 4;
+  const T ca = row == 0 ? T(1) : (row == 1 ? a0 : (row == 2 ? c2a : c3a));
+  const T cb = row == 0 ? T(0) : (row == 1 ? b0 : (row == 2 ? c2b : c3b));
+  b = a * cb + b;
+  a = a * ca;
+}
+template <typename T>
+__device__ __forceinline__ void scan_down(T &a, T &b, const int lane) {
+#define X3D_DN(S)                                                                                                      \
+  {                                                                                                                    \
+    const T an = dpp<0x100 + S>(a, T(1)), bn = dpp<0x100 + S>(b, T(0));                                               \
+    b = a * bn + b;                                                                                                    \
+    a = a * an;                                                                                                        \
+  }
+  X3D_DN(1) X3D_DN(2) X3D_DN(4) X3D_DN(8)
+#undef X3D_DN
+  const T a1 = readlane(a, 16), b1 = readlane(b, 16);
+  const T a2 = readlane(a, 32), b2 = readlane(b, 32);
+  const T a3 = readlane(a, 48), b3 = readlane(b, 48);
+  // carry into rows 2, 1, 0: rows r+1..3 composed
+  const T c1a = a2 * a3, c1b = a2 * b3 + b2;
+  const T c0a = a1 * c1a, c0b = a1 * c1b + b1;
+  const int row = lane >> 4;
+  const T ca = row == 3 ? T(1) : (row == 2 ? a3 : (row == 1 ? c1a : c0a));
+  const T cb = row == 3 ? T(0) : (row == 2 ? b3 : (row == 1 ? c1b : c0b));
+  b = a * cb + b;
+  a = a * ca;
+}
+
+// Periodic compact solve of one line held by a wavefront, V points per lane
+// (lane k owns points kV .. kV+V-1): local Thomas sweeps, carries across the
+// lanes by the affine-map scans, Sherman-Morrison closure (derive.f90:55-59)
+// with rden = 1 / den.
 template <typename T, int V>
 __device__ __forceinline__ void wave_solve(T (&v)[V], const T (&fs)[V], const T (&fw)[V], const T (&ffw)[V],
-                                           const T (&sm)[V], const T alfa, const T den, const int lane) {
+                                           const T (&sm)[V], const T alfa, const T rden, const int lane) {
   T prev = T(0), g = T(1);
 #pragma unroll
   for (int m = 0; m < V; ++m) {
@@ -443,16 +524,8 @@ __device__ __forceinline__ void wave_solve(T (&v)[V], const T (&fs)[V], const T 
   }
   {
     T a = g, bb = prev;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const T ap = __shfl_up(a, off), bp = __shfl_up(bb, off);
-      if (lane >= off) {
-        bb = a * bp + bb;
-        a = a * ap;
-      }
-    }
-    T cin = __shfl_up(bb, 1);
-    if (lane == 0) cin = T(0);
+    scan_up(a, bb, lane);
+    const T cin = dpp<0x138>(bb, T(0));   // wave_shr:1, lane 0 reads 0
     T gg = T(1);
 #pragma unroll
     for (int m = 0; m < V; ++m) {
@@ -469,16 +542,8 @@ __device__ __forceinline__ void wave_solve(T (&v)[V], const T (&fs)[V], const T 
   }
   {
     T a = h, bb = nxt;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const T an = __shfl_down(a, off), bn = __shfl_down(bb, off);
-      if (lane + off < 64) {
-        bb = a * bn + bb;
-        a = a * an;
-      }
-    }
-    T c = __shfl_down(bb, 1);
-    if (lane == 63) c = T(0);
+    scan_down(a, bb, lane);
+    const T c = dpp<0x130>(bb, T(0));     // wave_shl:1, lane 63 reads 0
     T hh = T(1);
 #pragma unroll
     for (int m = V - 1; m >= 0; --m) {
@@ -486,8 +551,8 @@ __device__ __forceinline__ void wave_solve(T (&v)[V], const T (&fs)[V], const T 
       v[m] += hh * c;
     }
   }
-  const T x0 = __shfl(v[0], 0), xn = __shfl(v[V - 1], 63);
-  const T sx = (x0 - alfa * xn) / den;
+  const T x0 = readlane(v[0], 0), xn = readlane(v[V - 1], 63);
+  const T sx = (x0 - alfa * xn) * rden;
 #pragma unroll
   for (int m = 0; m < V; ++m) v[m] = v[m] - sx * sm[m];
 }
@@ -905,79 +970,16 @@ __global__ void __launch_bounds__(256) xwave_kernel(const LineArgs<T> A, int nli
     }
     // window U(e0-HL .. e0+V+HR-1): own points plus neighbour lanes' points
     T v[W];
-#pragma unroll
-    for (int k = 0; k < W; ++k) {
-      const int d = k - HL;               // offset from e0
-      if (d >= 0 && d < V) {
-        v[k] = own[d];
-      } else {
-        const int lo = floor_div(d, V);    // compile-time after unrolling
-        const int el = d - lo * V;
-        v[k] = __shfl(own[el], (lane + lo) & 63);
-      }
-    }
-    // right-hand side in place
+    halo_window<T, V, HL, HR>(own, v, lane);
+    // right-hand side in place, then the solve
 #pragma unroll
     for (int m = 0; m < V; ++m) v[m] = interior_row<OP, T, W>(v, m + HL, A.s);
-    // forward elimination: local, then scan of (g, y_end) across lanes
-    T prev = T(0), g = T(1);
+    T r[V];
 #pragma unroll
-    for (int m = 0; m < V; ++m) {
-      prev = v[m] - prev * fs[m];
-      v[m] = prev;
-      g = -fs[m] * g;
-    }
-    {
-      T a = g, bb = prev;
+    for (int m = 0; m < V; ++m) r[m] = v[m];
+    wave_solve<T, V>(r, fs, fw, ffw, sm, A.alfa, A.rden, lane);
 #pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const T ap = __shfl_up(a, off), bp = __shfl_up(bb, off);
-        if (lane >= off) {
-          bb = a * bp + bb;
-          a = a * ap;
-        }
-      }
-      T cin = __shfl_up(bb, 1);
-      if (lane == 0) cin = T(0);
-      T gg = T(1);
-#pragma unroll
-      for (int m = 0; m < V; ++m) {
-        gg = -fs[m] * gg;
-        v[m] += gg * cin;
-      }
-    }
-    // back substitution: local, then reverse scan of (h, x_start)
-    T nxt = T(0), h = T(1);
-#pragma unroll
-    for (int m = V - 1; m >= 0; --m) {
-      nxt = fw[m] * v[m] - ffw[m] * nxt;
-      v[m] = nxt;
-      h = -ffw[m] * h;
-    }
-    {
-      T a = h, bb = nxt;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const T an = __shfl_down(a, off), bn = __shfl_down(bb, off);
-        if (lane + off < 64) {
-          bb = a * bn + bb;
-          a = a * an;
-        }
-      }
-      T c = __shfl_down(bb, 1);
-      if (lane == 63) c = T(0);
-      T hh = T(1);
-#pragma unroll
-      for (int m = V - 1; m >= 0; --m) {
-        hh = -ffw[m] * hh;
-        v[m] += hh * c;
-      }
-    }
-    // Sherman-Morrison correction (derive.f90:55-59)
-    const T x0 = __shfl(v[0], 0), xn = __shfl(v[V - 1], 63);
-    const T sx = (x0 - A.alfa * xn) / A.sm_den;
-#pragma unroll
-    for (int m = 0; m < V; ++m) v[m] = v[m] - sx * sm[m];
+    for (int m = 0; m < V; ++m) v[m] = r[m];
     // epilogue and store
     T res[V];
 #pragma unroll
@@ -1084,7 +1086,7 @@ int launch_line(const LineCall<T> &c, hipStream_t st) {
   a.u = c.u; a.u2 = c.u2; a.t = c.t; a.pp = c.pp;
   a.acc = c.acc; a.mul = c.mul; a.ac = c.ac; a.rc = c.rc;
   a.pk = c.tri.pk;
-  a.alfa = c.tri.alfa; a.sm_den = c.tri.sm_den;
+  a.alfa = c.tri.alfa; a.sm_den = c.tri.sm_den; a.rden = c.tri.rden;
   a.s = c.s;
   a.bc1 = c.bc1; a.bcn = c.bcn; a.npaire = c.npaire;
   const bool periodic = c.tri.periodic != 0;

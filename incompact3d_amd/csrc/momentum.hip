@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(256) mom_x_kernel(const MomArgs<T> A, int nlin
       T w[V];
 #pragma unroll
       for (int m = 0; m < V; ++m) w[m] = A.s.afi * (v[m + 3] - v[m + 1]) + A.s.bfi * (v[m + 4] - v[m]);
-      wave_solve<T, V>(w, fs1, fw1, ffw1, sm1, A.alfa1, A.den1, lane);
+      wave_solve<T, V>(w, fs1, fw1, ffw1, sm1, A.alfa1, A.rden1, lane);
 #pragma unroll
       for (int m = 0; m < V; ++m) r[m] = T(-0.5) * w[m];
     }
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(256) mom_x_kernel(const MomArgs<T> A, int nlin
       T w[V];
 #pragma unroll
       for (int m = 0; m < V; ++m) w[m] = A.s.afi * (v[m + 3] - v[m + 1]) + A.s.bfi * (v[m + 4] - v[m]);
-      wave_solve<T, V>(w, fs1, fw1, ffw1, sm1, A.alfa1, A.den1, lane);
+      wave_solve<T, V>(w, fs1, fw1, ffw1, sm1, A.alfa1, A.rden1, lane);
 #pragma unroll
       for (int m = 0; m < V; ++m) r[m] += T(-0.5) * (ud[m] * w[m]);
     }
@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(256) mom_x_kernel(const MomArgs<T> A, int nlin
       T w[V];
 #pragma unroll
       for (int m = 0; m < V; ++m) w[m] = interior_row<OP_DER2, T, V + 8>(v, m + 4, A.s);
-      wave_solve<T, V>(w, fs2, fw2, ffw2, sm2, A.alfa2, A.den2, lane);
+      wave_solve<T, V>(w, fs2, fw2, ffw2, sm2, A.alfa2, A.rden2, lane);
 #pragma unroll
       for (int m = 0; m < V; ++m) r[m] += A.xnu * w[m];
     }

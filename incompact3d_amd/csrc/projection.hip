@@ -203,9 +203,9 @@ __global__ void __launch_bounds__(256) proj_x_kernel(const ProjArgs<T> A, int nl
         for (int m = 0; m < V; ++m) wD[m] = interior_row<OPD, T, V + 8>(v, m + 4, A.s);
       }
     }
-    wave_solve<T, V>(wI[0], fsI, fwI, ffwI, smI, A.alfaI, A.denI, lane);
-    wave_solve<T, V>(wI[1], fsI, fwI, ffwI, smI, A.alfaI, A.denI, lane);
-    wave_solve<T, V>(wD, fsD, fwD, ffwD, smD, A.alfaD, A.denD, lane);
+    wave_solve<T, V>(wI[0], fsI, fwI, ffwI, smI, A.alfaI, A.rdenI, lane);
+    wave_solve<T, V>(wI[1], fsI, fwI, ffwI, smI, A.alfaI, A.rdenI, lane);
+    wave_solve<T, V>(wD, fsD, fwD, ffwD, smD, A.alfaD, A.rdenD, lane);
     if constexpr (SW == SW_DIV_X) {
       store_run<T, V>(A.out[0] + base, wD);
       store_run<T, V>(A.out[1] + base, wI[0]);
