@@ -90,11 +90,37 @@ def run_les_step(X, p, warmup, steps, stream):
     return ev0.elapsed_time(ev1) / steps
 
 
+def roofline_momentum_yz(X, p, stream, reps=10):
+    """Roofline of the step's dominant kernel, the fused y/z momentum sweep
+    (mom_yz_kernel, FP64): the six y/z sweeps of one step launched `reps`
+    times through x3d_solver_momentum_yz on the library stream, timed with
+    HIP events on that stream.  Algorithmic bytes per launch: the fields each
+    launch must move once (u_d if it differs from u_c, u_c, the accumulator
+    when present, the output), N*8 bytes each, averaged over the six launch
+    configurations (19 fields per step / 6)."""
+    import torch
+    s = X.Solver(p, 0)
+    s.init_tgv()
+    s.momentum_yz(2)
+    torch.cuda.synchronize()
+    per_call = _events_time(stream, lambda: s.momentum_yz(1), reps)   # ms per six launches
+    nbytes = s.momentum_yz(0)
+    s.close()
+    ms = per_call / 6.0
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    n = p.nx
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic("mom_yz_kernel<double, 8>", n),
+            "kernel": f"x3d::mom_yz_kernel<double, 8> (fused y/z momentum sweep, {n}^3 FP64, mean of the six "
+                      f"launch configurations of a step)",
+            "avg_ms": round(ms, 5), "algorithmic_bytes": int(nbytes)}
+
+
 def roofline_line_kernel(X, n, stream, reps=40):
-    """Average duration of the dominant kernel -- the FP64 line-solve kernel
-    of a first derivative along y on the n^3 field -- measured with HIP events
-    on the library stream.  Algorithmic bytes per launch: 2*N*8 (one read of
-    u, one write of du/dy)."""
+    """One compact solve on its own: a first derivative along y of an n^3
+    FP64 field (dery_00, seg_line_kernel), timed with HIP events on the
+    library stream.  Algorithmic bytes per launch: 2*N*8 (one read of u, one
+    write of du/dy)."""
     import torch
     u = torch.randn((n, n, n), dtype=torch.float64, device="cuda")
     t = torch.empty_like(u)
@@ -105,20 +131,21 @@ def roofline_line_kernel(X, n, stream, reps=40):
     algo = 2.0 * n ** 3 * 8
     achieved = algo / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(n),
-            "kernel": f"x3d::line_kernel<double, 0, 32, 1, true, false> (dery_00, {n}^3 FP64)",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic("seg_line_kernel<double, 0, 8, false>", n),
+            "kernel": f"x3d::seg_line_kernel<double, 0, 8, false> (dery_00, {n}^3 FP64)",
             "avg_ms": round(ms, 5), "algorithmic_bytes": int(algo)}
 
 
-def _traffic(n):
-    """HBM bytes per launch of the roofline kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+def _traffic(kernel, n):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size)."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        if int(d.get("n", -1)) == n:
-            return d.get("traffic_bytes")
+        k = d.get("kernels", {}).get(kernel)
+        if k and int(d.get("n", -1)) == n:
+            return k.get("traffic_bytes")
     except (OSError, ValueError):
         pass
     return None
@@ -282,7 +309,8 @@ def main():
     X.set_stream(stream.cuda_stream)
     if args.roofline_only:
         with torch.cuda.stream(stream):
-            roof = roofline_line_kernel(X, n, stream, reps=10)
+            roof = roofline_momentum_yz(X, p, stream, reps=2)
+            line = roofline_line_kernel(X, n, stream, reps=10)
             # calibration: one pencil copy (transpose_x_to_y) of n^3 FP64 words,
             # i.e. n^3*8 bytes read and n^3*8 bytes written
             a = torch.randn((n, n, n), dtype=torch.float64, device="cuda")
@@ -290,12 +318,13 @@ def main():
             X.transpose_x_to_y(a, b)
             torch.cuda.synchronize()
         X.finalize()
-        print(json.dumps({"roofline": roof, "copy_bytes_each_way": n ** 3 * 8}))
+        print(json.dumps({"roofline": roof, "line_solve": line, "copy_bytes_each_way": n ** 3 * 8}))
         return
     with torch.cuda.stream(stream):
         r64 = run_solver(X, p, 0, args.warmup, args.steps, stream) if args.only != "fp32" else None
         r32 = run_solver(X, p, 1, args.warmup, args.steps, stream) if args.only != "fp64" else None
-        roof = roofline_line_kernel(X, n, stream)
+        roof = roofline_momentum_yz(X, p, stream)
+        line = roofline_line_kernel(X, n, stream)
         les_ms = run_les_step(X, p, args.warmup, args.steps, stream) if args.only == "both" else None
     npts = n ** 3
     main_r = r64 if r64 is not None else r32
@@ -317,6 +346,7 @@ def main():
         "config": {"workload": f"TGV {n}^3 triply periodic, Re=1600, AB3 dt=0.002, 6th-order compact, "
                                f"spectral Poisson", "grid": [n, n, n], "precision": "fp64"},
         "roofline": roof,
+        "line_solve_roofline": line,
     }
     if r64 is not None:
         out["fp64"] = {"ms_per_step": round(r64["event_ms_per_step"], 4),

@@ -259,6 +259,14 @@ int x3d_solver_tgv_stats(x3d_solver *s, double out4[4]);
  * (navier.f90:346-361 with nlock=2 semantics, FP64). */
 int x3d_solver_divergence_stats(x3d_solver *s, double out2[2]);
 int x3d_solver_itime(x3d_solver *s, int *itime);
+/* Instrumentation, no counterpart in the reference: enqueue nrep times the
+ * six fused y/z momentum sweeps of one step (z then y for each component,
+ * on the current velocity, into the solver's work buffer; the velocity and
+ * the AB3 history are not touched) on the library stream, without
+ * synchronizing.  *bytes_per_launch receives the HBM bytes one launch moves
+ * on average (u_d when it differs from u_c, u_c, the accumulator when
+ * present, the output).  Fused solvers only. */
+int x3d_solver_momentum_yz(x3d_solver *s, int nrep, double *bytes_per_launch);
 
 /* ---- LES subgrid stress (les_models.f90) ----------------------------------
  * model 0: none (DNS); 1: Smagorinsky, jles=1 of compute_SGS

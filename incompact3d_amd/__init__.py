@@ -195,6 +195,14 @@ class Solver:
         check(_lib.lib().x3d_solver_divergence_stats(self._h, out), "divergence_stats")
         return tuple(out)
 
+    def momentum_yz(self, nrep: int = 1) -> float:
+        """x3d_solver_momentum_yz: enqueue nrep times the six fused y/z
+        momentum sweeps of one step on the library stream; returns the HBM
+        bytes one launch moves on average."""
+        b = C.c_double(0.0)
+        check(_lib.lib().x3d_solver_momentum_yz(self._h, int(nrep), C.byref(b)), "momentum_yz")
+        return b.value
+
     @property
     def itime(self):
         v = C.c_int(0)

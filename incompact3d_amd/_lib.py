@@ -91,6 +91,7 @@ def _declare(lib):
     lib.x3d_solver_tgv_stats.argtypes = [P, C.POINTER(C.c_double)]
     lib.x3d_solver_divergence_stats.argtypes = [P, C.POINTER(C.c_double)]
     lib.x3d_solver_itime.argtypes = [P, IP]
+    lib.x3d_solver_momentum_yz.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
     LLP = C.POINTER(C.c_longlong)
     lib.x3d_solver_set_les.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, P, C.c_double, C.c_double]
     lib.x3d_solver_sgs.argtypes = [P, P, P, P]

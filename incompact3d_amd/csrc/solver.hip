@@ -215,6 +215,31 @@ int gradp_correct(const Context &c, x3d_solver *S, T *const u[3], T *const W[4])
   return X3D_OK;
 }
 
+// The six y/z momentum sweeps of step_once (no SGS force), nrep times, into
+// the work buffer W[0]; *bytes = algorithmic bytes per launch on average.
+template <typename T>
+int momentum_yz(Context &c, x3d_solver *S, int nrep, double *bytes) {
+  const int nx = S->nx, ny = S->ny, nz = S->nz;
+  const T *cu[3];
+  for (int i = 0; i < 3; ++i) cu[i] = sizeof(T) == 8 ? (const T *)S->u64[i] : (const T *)S->u32[i];
+  T *W0 = (T *)S->w[0];
+  double fields = 0;
+  for (int comp = 0; comp < 3; ++comp) {
+    fields += (comp == 2 ? 1 : 2) + 1;        // z: u_z (and u_c), out
+    fields += (comp == 1 ? 1 : 2) + 2;        // y: u_y (and u_c), acc, out
+  }
+  *bytes = fields / 6.0 * (double)S->n * sizeof(T);
+  for (int r = 0; r < nrep; ++r) {
+    for (int comp = 0; comp < 3; ++comp) {
+      X3D_TRY(launch_momentum<T>(c, 2, comp, cu, nullptr, W0, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0,
+                                 0, nx, ny, nz, c.stream));
+      X3D_TRY(launch_momentum<T>(c, 1, comp, cu, W0, W0, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, 0,
+                                 nx, ny, nz, c.stream));
+    }
+  }
+  return X3D_OK;
+}
+
 template <typename T>
 int step_once(Context &c, x3d_solver *S) {
   const int nx = S->nx, ny = S->ny, nz = S->nz;
@@ -578,6 +603,15 @@ int x3d_solver_les_fallbacks(x3d_solver *S, long long *count) {
   }
   *count = h;
   return X3D_OK;
+}
+
+int x3d_solver_momentum_yz(x3d_solver *S, int nrep, double *bytes_per_launch) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || nrep < 0 || !bytes_per_launch) return fail(X3D_ERR_ARG, "bad arguments");
+  if (!S->fused) return fail(X3D_ERR_UNSUPPORTED, "momentum sweeps: the solver does not use the fused path");
+  if (S->precision == 0) return momentum_yz<double>(*c, S, nrep, bytes_per_launch);
+  return momentum_yz<float>(*c, S, nrep, bytes_per_launch);
 }
 
 int x3d_solver_itime(x3d_solver *S, int *itime) {
