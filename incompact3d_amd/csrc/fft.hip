@@ -7,7 +7,12 @@
 // (poisson.f90:1706-1783).  poisson_010 (poisson.f90:665-1013) is the same
 // with a DCT-II along y computed, as in the reference, from an even/odd
 // reordering plus a complex FFT (Makhoul's algorithm), and with the
-// (nx/2+1, :, nz/2+1) line zeroed.
+// (nx/2+1, :, nz/2+1) line zeroed.  poisson_11x (poisson.f90:1019-1465,
+// istret=0) adds the same even/odd reorder along x and, for bcz=1, along z;
+// along x, the real-to-complex direction here, the half spectrum is shifted by
+// e^{-i pi k/(2 nx)}, after which its real part holds cosine mode k and its
+// imaginary part mode nx-k, so the two are divided by K at those two modes
+// (the real/imaginary split of waves(), poisson.f90:1740-1800).
 //
 // Passes over the half spectrum cw (nxh = nx/2+1, ny, nz), kx fastest:
 //   1 r2c along x (real line packed as nx/2 complex points, FFT, unpack)
@@ -49,14 +54,17 @@ struct PoissonTables {
   const cplx<T> *twx, *twy, *twz;   // exp(-2 pi i m / N) for N = nx/2, ny, nz
   const cplx<T> *rtx;               // exp(-2 pi i k / nx), k = 0..nx/2 (r2c unpack)
   const cplx<T> *dcty;              // exp(-i pi k / (2 ny)), k = 0..ny-1 (DCT along y)
-  const double *kx2, *tx;           // modified wavenumber^2 and transfer fn, per kx (nxh)
+  const cplx<T> *dctx;              // exp(-i pi k / (2 nx)), k = 0..nx/2 (DCT along x)
+  const cplx<T> *dctz;              // exp(-i pi k / (2 nz)), k = 0..nz-1 (DCT along z)
+  const double *kx2, *tx;           // modified wavenumber^2 and transfer fn, per kx (nxh; nx+1 modes with xdct)
   const double *ky2, *ty;           // per ky (ny)
   const double *kz2, *tz;           // per kz (nz)
 };
 
 struct PoissonPlan {
   int nx, ny, nz, nxh;              // pressure-grid sizes (ny = nym when y is not periodic)
-  int ydct;                         // 1 when y is non-periodic (poisson_010)
+  int ydct;                         // 1 when y is non-periodic (poisson_010, poisson_11x)
+  int xdct, zdct;                   // 1 when x / z are non-periodic (poisson_11x)
   double scale;                     // 1/(nx*ny*nz)
   void *tab64 = nullptr, *tab32 = nullptr;
   PoissonTables<double> t64;
@@ -268,11 +276,14 @@ struct SShape {
   static constexpr size_t lds() { return sizeof(cplx<T>) * ((size_t)TL * N + 2 * N); }
 };
 
+// row of point p under the even/odd reorder of poisson.f90:688-697 / :997-1008
+__device__ __forceinline__ int dct_row(int p, int N) { return p < (N >> 1) ? 2 * p : 2 * (N - 1 - p) + 1; }
+
 // ---------------------------------------------------------------- pass 1: r2c along x
 // X_k = (Z_k + conj Z_{M-k})/2 - i/2 e^{-2 pi i k/nx} (Z_k - conj Z_{M-k}), k = 0..M
 template <typename T, int M>
 __global__ void __launch_bounds__(FFT_THREADS) r2c_x_kernel(const T *__restrict__ in, cplx<T> *__restrict__ out,
-                                                             int nlines, PoissonTables<T> tb) {
+                                                             int nlines, int xdct, PoissonTables<T> tb) {
   using S = XShape<M>;
   constexpr int E = LineShape<M>::E, NT = S::NT, TL = S::TL;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -284,9 +295,20 @@ __global__ void __launch_bounds__(FFT_THREADS) r2c_x_kernel(const T *__restrict_
   stage_table(tw, tb.twx, M);
   stage_table(rt, tb.rtx, M + 1);
   cplx<T> x[E];
-  const cplx<T> *src = reinterpret_cast<const cplx<T> *>(in + (size_t)line * 2 * M);
+  if (xdct) {
+    // even/odd reorder along x (poisson.f90:1086-1095): point p of the FFT
+    // input is row dct_row(p) of the line
+    const T *src = in + (size_t)line * 2 * M;
 #pragma unroll
-  for (int m = 0; m < E; ++m) x[m] = valid ? src[t + m * NT] : cplx<T>{T(0), T(0)};
+    for (int m = 0; m < E; ++m) {
+      const int q = t + m * NT;
+      x[m] = valid ? cplx<T>{src[dct_row(2 * q, 2 * M)], src[dct_row(2 * q + 1, 2 * M)]} : cplx<T>{T(0), T(0)};
+    }
+  } else {
+    const cplx<T> *src = reinterpret_cast<const cplx<T> *>(in + (size_t)line * 2 * M);
+#pragma unroll
+    for (int m = 0; m < E; ++m) x[m] = valid ? src[t + m * NT] : cplx<T>{T(0), T(0)};
+  }
   __syncthreads();
   const LdLine<T> ld{buf, l * S::LP};
   fft_stages<T, M, false, 1>(x, t, ld, tw);
@@ -304,9 +326,14 @@ __global__ void __launch_bounds__(FFT_THREADS) r2c_x_kernel(const T *__restrict_
       const cplx<T> zm = cconj(ld.load(k == 0 ? 0 : M - k));
       const cplx<T> e = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y + zm.y)};
       const cplx<T> o = {T(0.5) * (zk.y - zm.y), -T(0.5) * (zk.x - zm.x)};   // (Z_k - conj Z_{M-k}) / (2i)
-      dst[k] = cadd(e, cmul(rt[k], o));
+      cplx<T> v = cadd(e, cmul(rt[k], o));
+      // cosine transform along x: shift by e^{-i pi k/(2 nx)} (poisson.f90:1116-1125)
+      if (xdct) v = cmul(tb.dctx[k], v);
+      dst[k] = v;
       if (k == 0) {   // k = M uses Z_0 on both sides
-        dst[M] = cadd(e, cmul(rt[M], o));
+        cplx<T> vm = cadd(e, cmul(rt[M], o));
+        if (xdct) vm = cmul(tb.dctx[M], vm);
+        dst[M] = vm;
       }
     }
   }
@@ -316,7 +343,7 @@ __global__ void __launch_bounds__(FFT_THREADS) r2c_x_kernel(const T *__restrict_
 // Z_k = (X_k + conj X_{M-k}) + i e^{+2 pi i k/nx} (X_k - conj X_{M-k}),  k < M
 template <typename T, int M>
 __global__ void __launch_bounds__(FFT_THREADS) c2r_x_kernel(const cplx<T> *__restrict__ in, T *__restrict__ out,
-                                                             int nlines, PoissonTables<T> tb) {
+                                                             int nlines, int xdct, PoissonTables<T> tb) {
   using S = XShape<M>;
   constexpr int E = LineShape<M>::E, NT = S::NT, TL = S::TL;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -334,6 +361,11 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_x_kernel(const cplx<T> *__res
     const int k = t + m * NT;
     xk[m] = valid ? src[k] : cplx<T>{T(0), T(0)};
     xm[m] = valid ? src[M - k] : cplx<T>{T(0), T(0)};
+    if (xdct) {
+      // undo the cosine-transform shift: e^{+i pi k/(2 nx)} (poisson.f90:1428-1437)
+      xk[m] = cmul(cconj(tb.dctx[k]), xk[m]);
+      xm[m] = cmul(cconj(tb.dctx[M - k]), xm[m]);
+    }
   }
   __syncthreads();
   cplx<T> x[E];
@@ -347,7 +379,16 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_x_kernel(const cplx<T> *__res
   }
   const LdLine<T> ld{buf, l * S::LP};
   fft_stages<T, M, true, 1>(x, t, ld, tw);
-  if (valid) {
+  if (valid && xdct) {
+    // inverse reorder along x (poisson.f90:1452-1460)
+    T *dst = out + (size_t)line * 2 * M;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int q = t + m * NT;
+      dst[dct_row(2 * q, 2 * M)] = x[m].x;
+      dst[dct_row(2 * q + 1, 2 * M)] = x[m].y;
+    }
+  } else if (valid) {
     cplx<T> *dst = reinterpret_cast<cplx<T> *>(out + (size_t)line * 2 * M);
 #pragma unroll
     for (int m = 0; m < E; ++m) dst[t + m * NT] = x[m];
@@ -357,13 +398,14 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_x_kernel(const cplx<T> *__res
 // ---------------------------------------------------------------- passes 2-4: strided lines
 enum { FFT_Y_FWD = 0, FFT_Y_INV = 1, FFT_Z_SOLVE = 2 };
 
-// row of point p under the even/odd reorder of poisson.f90:688-697 / :997-1008
-__device__ __forceinline__ int dct_row(int p, int N) { return p < (N >> 1) ? 2 * p : 2 * (N - 1 - p) + 1; }
 
-template <typename T, int N, int MODE>
+
+// COS: the poisson_11x z pass (cosine transform along x, and along z when
+// zdct), compiled separately so the periodic passes keep their registers.
+template <typename T, int N, int MODE, bool COS = false>
 __global__ void __launch_bounds__(FFT_THREADS) strided_kernel(cplx<T> *__restrict__ cw, int nxh, int ny, int nz,
-                                                               int ydct, double scale, int zero_line,
-                                                               PoissonTables<T> tb) {
+                                                               int ydct, double scale, int zero_line, int xdct,
+                                                               int zdct, PoissonTables<T> tb) {
   using S = SShape<N>;
   constexpr int E = LineShape<N>::E, NT = S::NT, TL = S::TL;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -376,8 +418,10 @@ __global__ void __launch_bounds__(FFT_THREADS) strided_kernel(cplx<T> *__restric
   const size_t se = MODE == FFT_Z_SOLVE ? (size_t)nxh * ny : (size_t)nxh;
   const size_t base = MODE == FFT_Z_SOLVE ? (size_t)b * nxh + a : (size_t)b * nxh * ny + a;
   stage_table(tw, MODE == FFT_Z_SOLVE ? tb.twz : tb.twy, N);
-  if (MODE != FFT_Z_SOLVE && ydct) stage_table(dct, tb.dcty, N);
-  const bool reorder_in = MODE == FFT_Y_FWD && ydct;
+  // the cosine transform of this pass: y in the y passes, z in the z pass
+  const bool dct_here = MODE == FFT_Z_SOLVE ? (COS && zdct != 0) : ydct != 0;
+  if (dct_here) stage_table(dct, MODE == FFT_Z_SOLVE ? tb.dctz : tb.dcty, N);
+  const bool reorder_in = (MODE == FFT_Y_FWD || MODE == FFT_Z_SOLVE) && dct_here;
   cplx<T> x[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -386,7 +430,7 @@ __global__ void __launch_bounds__(FFT_THREADS) strided_kernel(cplx<T> *__restric
     x[m] = valid ? cw[base + (size_t)row * se] : cplx<T>{T(0), T(0)};
   }
   if constexpr (MODE == FFT_Y_INV) {
-    if (ydct) {
+    if (dct_here) {
       // DCT-III input side: V_k = e^{+i pi k/2N} (C_k - i C_{N-k}), V_0 = C_0
       // (poisson.f90:924-943); C_{N-k} is read from its own row
       cplx<T> cm[E];
@@ -407,9 +451,9 @@ __global__ void __launch_bounds__(FFT_THREADS) strided_kernel(cplx<T> *__restric
   const LdRows<T> ld{buf, TL, l};
   fft_stages<T, N, MODE == FFT_Y_INV, 1>(x, t, ld, tw);
 
-  if (MODE == FFT_Y_FWD && ydct) {
+  if ((MODE == FFT_Y_FWD || MODE == FFT_Z_SOLVE) && dct_here) {
     // DCT-II output side: C_k = (e^{-i pi k/2N} V_k + e^{+i pi k/2N} V_{N-k}) / 2,
-    // C_0 = V_0 (poisson.f90:761-781); V_{N-k} belongs to another thread
+    // C_0 = V_0 (poisson.f90:761-781, :1139-1160); V_{N-k} belongs to another thread
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < E; ++m) ld.store(t + m * NT, x[m]);
@@ -427,8 +471,25 @@ __global__ void __launch_bounds__(FFT_THREADS) strided_kernel(cplx<T> *__restric
   }
 
   if (MODE == FFT_Z_SOLVE) {
-    // divide by -K in FP64 (poisson.f90:362-371 / :795-819)
-    if (valid) {
+    // divide by -K in FP64 (poisson.f90:362-371 / :795-819 / :1191-1207).
+    // With a cosine transform along x the real part is mode kx and the
+    // imaginary part mode nx-kx, each divided by its own K.
+    if (valid && COS) {
+      const double ky2 = tb.ky2[b], tyv = tb.ty[b];
+      const int am = 2 * (nxh - 1) - a;
+      const double txr = tb.tx[a], kx2r = tb.kx2[a], txi = tb.tx[am], kx2i = tb.kx2[am];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int kz = t + m * NT;
+        const double tzv = tb.tz[kz], kz2 = tb.kz2[kz];
+        const double yz2 = (tyv * tzv) * (tyv * tzv);
+        const double Kr = kx2r * yz2 + ky2 * (txr * tzv) * (txr * tzv) + kz2 * (txr * tyv) * (txr * tyv);
+        const double Ki = kx2i * yz2 + ky2 * (txi * tzv) * (txi * tzv) + kz2 * (txi * tyv) * (txi * tyv);
+        const double fr = Kr < 1.0e-16 ? 0.0 : -scale / Kr;
+        const double fi = Ki < 1.0e-16 ? 0.0 : -scale / Ki;
+        x[m] = cplx<T>{(T)((double)x[m].x * fr), (T)((double)x[m].y * fi)};
+      }
+    } else if (valid) {
       const double ky2 = tb.ky2[b], tyv = tb.ty[b];
       const double txv = tb.tx[a], kx2 = tb.kx2[a];
 #pragma unroll
@@ -442,11 +503,31 @@ __global__ void __launch_bounds__(FFT_THREADS) strided_kernel(cplx<T> *__restric
         x[m] = cplx<T>{(T)((double)x[m].x * f), (T)((double)x[m].y * f)};
       }
     }
+    if (COS && dct_here) {
+      // DCT-III input side along z: V_k = e^{+i pi k/2N} (C_k - i C_{N-k}),
+      // V_0 = C_0 (poisson.f90:1330-1351); C_{N-k} belongs to another thread
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < E; ++m) ld.store(t + m * NT, x[m]);
+      __syncthreads();
+      cplx<T> cm[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + m * NT;
+        cm[m] = k > 0 ? ld.load(N - k) : cplx<T>{T(0), T(0)};
+      }
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + m * NT;
+        x[m] = cmul(cconj(dct[k]), cplx<T>{x[m].x + cm[m].y, x[m].y - cm[m].x});
+      }
+      __syncthreads();
+    }
     fft_stages<T, N, true, 1>(x, t, ld, tw);
   }
 
   if (valid) {
-    const bool reorder_out = MODE == FFT_Y_INV && ydct;
+    const bool reorder_out = (MODE == FFT_Y_INV || MODE == FFT_Z_SOLVE) && dct_here;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int p = t + m * NT;
@@ -506,6 +587,24 @@ static void wave_1d(const DirCoef &D, int bc, std::vector<double> &k2, std::vect
   }
 }
 
+// k^2 and the transfer function at mode m of a non-periodic direction
+// (w = pi m / nm), the formulas of wave_1d's bc=1 branch, for m = nm.
+static void wave_1d_mode(const DirCoef &D, int m, std::vector<double> &k2, std::vector<double> &tf) {
+  const double pi = std::acos(-1.0);
+  const Sten<double> &c = D.s;
+  const double d = D.d, L = D.len;
+  const double w = pi * m / D.nm;
+  double wp = c.aci * 2.0 * d * std::sin(w * 0.5) + (c.bci * 2.0 * d) * std::sin(1.5 * w);
+  wp = wp / (1.0 + 2.0 * c.alcai * std::cos(w));
+  const double kk = D.nm * wp / L, es = D.nm * w / L;
+  const double r = es * d;
+  const double tt = 2.0 * (c.bici * std::cos(r * 1.5) + c.cici * std::cos(r * 2.5) + c.dici * std::cos(r * 3.5));
+  const double tt1 = 2.0 * c.aici * std::cos(r * 0.5);
+  const double t1 = 1.0 + 2.0 * c.ailcai * std::cos(r);
+  k2.assign(1, kk * kk);
+  tf.assign(1, (tt1 + tt) / t1);
+}
+
 template <typename T>
 static void append_cplx(std::vector<unsigned char> &blob, const std::vector<double> &re,
                         const std::vector<double> &im, size_t *off) {
@@ -561,6 +660,21 @@ static int build_tables(const PoissonPlan &P, const std::vector<double> *tabs, v
     im[k] = -std::sin((double)k * pi * 0.5 / (double)P.ny);
   }
   append_cplx<T>(blob, re, im, &o_dct);
+  size_t o_dctx, o_dctz;
+  re.assign(P.nx / 2 + 1, 0.0);
+  im.assign(P.nx / 2 + 1, 0.0);
+  for (int k = 0; k <= P.nx / 2; ++k) {
+    re[k] = std::cos((double)k * pi * 0.5 / (double)P.nx);
+    im[k] = -std::sin((double)k * pi * 0.5 / (double)P.nx);
+  }
+  append_cplx<T>(blob, re, im, &o_dctx);
+  re.assign(P.nz, 0.0);
+  im.assign(P.nz, 0.0);
+  for (int k = 0; k < P.nz; ++k) {
+    re[k] = std::cos((double)k * pi * 0.5 / (double)P.nz);
+    im[k] = -std::sin((double)k * pi * 0.5 / (double)P.nz);
+  }
+  append_cplx<T>(blob, re, im, &o_dctz);
   for (int i = 0; i < 6; ++i) append_real(blob, tabs[i], &o_r[i]);
   void *d = nullptr;
   X3D_HIP(hipMalloc(&d, blob.size()));
@@ -572,6 +686,8 @@ static int build_tables(const PoissonPlan &P, const std::vector<double> *tabs, v
   out->twz = reinterpret_cast<const cplx<T> *>(b + o_twz);
   out->rtx = reinterpret_cast<const cplx<T> *>(b + o_rtx);
   out->dcty = reinterpret_cast<const cplx<T> *>(b + o_dct);
+  out->dctx = reinterpret_cast<const cplx<T> *>(b + o_dctx);
+  out->dctz = reinterpret_cast<const cplx<T> *>(b + o_dctz);
   out->kx2 = reinterpret_cast<const double *>(b + o_r[0]);
   out->tx = reinterpret_cast<const double *>(b + o_r[1]);
   out->ky2 = reinterpret_cast<const double *>(b + o_r[2]);
@@ -583,13 +699,20 @@ static int build_tables(const PoissonPlan &P, const std::vector<double> *tabs, v
 
 int poisson_setup(Context &c) {
   const DirCoef &X = c.dir[0], &Y = c.dir[1], &Z = c.dir[2];
-  if (X.bc1 != 0 || Z.bc1 != 0) return fail(X3D_ERR_UNSUPPORTED, "Poisson: x and z must be periodic (poisson_100/11x are not provided)");
+  // dispatch of decomp_2d_poisson_init (poisson.f90:98-108): 000, 010 and
+  // 11x (x and y non-periodic, z either); poisson_100 is not provided
+  if (X.bc1 != 0 && Y.bc1 == 0)
+    return fail(X3D_ERR_UNSUPPORTED, "Poisson: x non-periodic with y periodic (poisson_100) is not provided");
+  if (X.bc1 == 0 && Z.bc1 != 0)
+    return fail(X3D_ERR_UNSUPPORTED, "Poisson: z non-periodic needs x and y non-periodic (poisson_11x)");
   auto *P = new PoissonPlan();
-  P->nx = X.n;
-  P->ny = Y.nm;
-  P->nz = Z.n;
-  P->nxh = P->nx / 2 + 1;
+  P->xdct = X.bc1 != 0;
   P->ydct = Y.bc1 != 0;
+  P->zdct = Z.bc1 != 0;
+  P->nx = P->xdct ? X.nm : X.n;
+  P->ny = Y.nm;
+  P->nz = P->zdct ? Z.nm : Z.n;
+  P->nxh = P->nx / 2 + 1;
   P->scale = 1.0 / ((double)P->nx * (double)P->ny * (double)P->nz);
   const int lx = ilog2(P->nx), ly = ilog2(P->ny), lz = ilog2(P->nz);
   if (lx < 4 || ly < 3 || lz < 3 || P->nx > 2048 || P->ny > 1024 || P->nz > 1024) {
@@ -597,11 +720,19 @@ int poisson_setup(Context &c) {
     return fail(X3D_ERR_UNSUPPORTED, "Poisson: pressure-grid sizes must be powers of two (nx 16..2048, ny/nz 8..1024)");
   }
   std::vector<double> tabs[6];
-  wave_1d(X, 0, tabs[0], tabs[1]);
+  wave_1d(X, P->xdct ? 1 : 0, tabs[0], tabs[1]);
   wave_1d(Y, P->ydct ? 1 : 0, tabs[2], tabs[3]);
-  wave_1d(Z, 0, tabs[4], tabs[5]);
-  tabs[0].resize(P->nxh);
-  tabs[1].resize(P->nxh);
+  wave_1d(Z, P->zdct ? 1 : 0, tabs[4], tabs[5]);
+  if (P->xdct) {
+    // the imaginary part of kx carries mode nx-kx: modes 0..nx (poisson.f90:1646-1655)
+    std::vector<double> k2e, tfe;
+    wave_1d_mode(X, P->nx, k2e, tfe);
+    tabs[0].push_back(k2e[0]);
+    tabs[1].push_back(tfe[0]);
+  } else {
+    tabs[0].resize(P->nxh);
+    tabs[1].resize(P->nxh);
+  }
   int r = build_tables<double>(*P, tabs, &P->tab64, &P->t64);
   if (r == X3D_OK) r = build_tables<float>(*P, tabs, &P->tab32, &P->t32);
   if (r != X3D_OK) {
@@ -624,20 +755,27 @@ void poisson_free(Context &c) {
 }
 
 template <typename T, int M>
-static void launch_x(bool fwd, T *rhs, cplx<T> *cw, int nlines, const PoissonTables<T> &tb, hipStream_t st) {
+static void launch_x(bool fwd, T *rhs, cplx<T> *cw, int nlines, int xdct, const PoissonTables<T> &tb,
+                     hipStream_t st) {
   using S = XShape<M>;
   const dim3 grid((nlines + S::TL - 1) / S::TL), block(S::TL * S::NT);
   const size_t sh = S::template lds<T>();
-  if (fwd) hipLaunchKernelGGL((r2c_x_kernel<T, M>), grid, block, sh, st, rhs, cw, nlines, tb);
-  else hipLaunchKernelGGL((c2r_x_kernel<T, M>), grid, block, sh, st, cw, rhs, nlines, tb);
+  if (fwd) hipLaunchKernelGGL((r2c_x_kernel<T, M>), grid, block, sh, st, rhs, cw, nlines, xdct, tb);
+  else hipLaunchKernelGGL((c2r_x_kernel<T, M>), grid, block, sh, st, cw, rhs, nlines, xdct, tb);
 }
 
 template <typename T, int N, int MODE>
 static void launch_s(const PoissonPlan &P, cplx<T> *cw, const PoissonTables<T> &tb, hipStream_t st) {
   using S = SShape<N>;
   const dim3 grid((P.nxh + S::TL - 1) / S::TL, MODE == FFT_Z_SOLVE ? P.ny : P.nz), block(S::TL * S::NT);
-  hipLaunchKernelGGL((strided_kernel<T, N, MODE>), grid, block, S::template lds<T>(), st, cw, P.nxh, P.ny, P.nz,
-                     P.ydct, P.scale, MODE == FFT_Z_SOLVE ? P.ydct : 0, tb);
+  // poisson_010 zeroes the (nx/2+1, :, nz/2+1) line (poisson.f90:902-908); poisson_11x does not
+  const int zero_line = MODE == FFT_Z_SOLVE && P.ydct && !P.xdct;
+  if (MODE == FFT_Z_SOLVE && P.xdct)
+    hipLaunchKernelGGL((strided_kernel<T, N, MODE, true>), grid, block, S::template lds<T>(), st, cw, P.nxh, P.ny,
+                       P.nz, P.ydct, P.scale, zero_line, P.xdct, P.zdct, tb);
+  else
+    hipLaunchKernelGGL((strided_kernel<T, N, MODE>), grid, block, S::template lds<T>(), st, cw, P.nxh, P.ny, P.nz,
+                       P.ydct, P.scale, zero_line, P.xdct, P.zdct, tb);
 }
 
 template <typename T, int MODE>
@@ -656,17 +794,17 @@ static bool dispatch_s(int N, const PoissonPlan &P, cplx<T> *cw, const PoissonTa
 }
 
 template <typename T>
-static bool dispatch_x(bool fwd, int M, T *rhs, cplx<T> *cw, int nlines, const PoissonTables<T> &tb,
+static bool dispatch_x(bool fwd, int M, T *rhs, cplx<T> *cw, int nlines, int xdct, const PoissonTables<T> &tb,
                        hipStream_t st) {
   switch (M) {
-    case 8: launch_x<T, 8>(fwd, rhs, cw, nlines, tb, st); return true;
-    case 16: launch_x<T, 16>(fwd, rhs, cw, nlines, tb, st); return true;
-    case 32: launch_x<T, 32>(fwd, rhs, cw, nlines, tb, st); return true;
-    case 64: launch_x<T, 64>(fwd, rhs, cw, nlines, tb, st); return true;
-    case 128: launch_x<T, 128>(fwd, rhs, cw, nlines, tb, st); return true;
-    case 256: launch_x<T, 256>(fwd, rhs, cw, nlines, tb, st); return true;
-    case 512: launch_x<T, 512>(fwd, rhs, cw, nlines, tb, st); return true;
-    case 1024: launch_x<T, 1024>(fwd, rhs, cw, nlines, tb, st); return true;
+    case 8: launch_x<T, 8>(fwd, rhs, cw, nlines, xdct, tb, st); return true;
+    case 16: launch_x<T, 16>(fwd, rhs, cw, nlines, xdct, tb, st); return true;
+    case 32: launch_x<T, 32>(fwd, rhs, cw, nlines, xdct, tb, st); return true;
+    case 64: launch_x<T, 64>(fwd, rhs, cw, nlines, xdct, tb, st); return true;
+    case 128: launch_x<T, 128>(fwd, rhs, cw, nlines, xdct, tb, st); return true;
+    case 256: launch_x<T, 256>(fwd, rhs, cw, nlines, xdct, tb, st); return true;
+    case 512: launch_x<T, 512>(fwd, rhs, cw, nlines, xdct, tb, st); return true;
+    case 1024: launch_x<T, 1024>(fwd, rhs, cw, nlines, xdct, tb, st); return true;
     default: return false;
   }
 }
@@ -684,11 +822,11 @@ int poisson_solve(Context &c, T *rhs, hipStream_t st) {
   else tb = P->t32;
   const int nlines = P->ny * P->nz, M = P->nx / 2;
   // sizes were checked by poisson_setup: every dispatch below finds its case
-  bool ok = dispatch_x<T>(true, M, rhs, cw, nlines, tb, st);
+  bool ok = dispatch_x<T>(true, M, rhs, cw, nlines, P->xdct, tb, st);
   ok = ok && dispatch_s<T, FFT_Y_FWD>(P->ny, *P, cw, tb, st);
   ok = ok && dispatch_s<T, FFT_Z_SOLVE>(P->nz, *P, cw, tb, st);
   ok = ok && dispatch_s<T, FFT_Y_INV>(P->ny, *P, cw, tb, st);
-  ok = ok && dispatch_x<T>(false, M, rhs, cw, nlines, tb, st);
+  ok = ok && dispatch_x<T>(false, M, rhs, cw, nlines, P->xdct, tb, st);
   if (!ok) return fail(X3D_ERR_UNSUPPORTED, "Poisson: no FFT kernel for this size");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("poisson launch: ") + hipGetErrorString(e));

@@ -898,6 +898,152 @@ def poisson_010(S: Schemes, rhs):
     return out
 
 
+def _mode_k2_t(c, d, nm, L, w):
+    """k'^2 and interpolation transfer at angle w of a non-periodic direction
+    (the bc=1 formulas of waves(), poisson.f90:1583-1596 / 1646-1655)."""
+    wp = c["aci"] * 2.0 * d * math.sin(w * 0.5) + (c["bci"] * 2.0 * d) * math.sin(1.5 * w)
+    wp = wp / (1.0 + 2.0 * c["alcai"] * math.cos(w))
+    r = (nm * w / L) * d
+    tt = 2.0 * (c["bici"] * math.cos(r * 1.5) + c["cici"] * math.cos(r * 2.5) + c["dici"] * math.cos(r * 3.5))
+    tt1 = 2.0 * c["aici"] * math.cos(r * 0.5)
+    t1 = 1.0 + 2.0 * c["ailcai"] * math.cos(r)
+    return (nm * wp / L) ** 2, (tt1 + tt) / t1
+
+
+def waves_kxyz_11x(S: Schemes):
+    """kxyz for bcx = bcy = 1 (poisson.f90:1740-1800) on the half spectrum in
+    z: (K_re, K_im), shape (nz/2+1, ny, nx) of the pressure grid.  With bcz=1
+    the real part of z index k is cosine mode k and the imaginary part mode
+    nzm-k (ezs, zk2 of poisson.f90:1646-1655); with bcz=0 both are mode k."""
+    p = S.p
+    bcz = 0 if p.nclz else 1
+    kx2, Tx = _wave_1d(S.st["x"], p.dx, p.nx, p.nxm, p.xlx, 1)
+    ky2, Ty = _wave_1d(S.st["y"], p.dy, p.ny, p.nym, p.yly, 1)
+    nz = p.nzm if bcz else p.nz
+    nzh = nz // 2 + 1
+    if bcz:
+        kre = np.zeros(nzh); tre = np.zeros(nzh); kim = np.zeros(nzh); tim = np.zeros(nzh)
+        for k in range(nzh):
+            kre[k], tre[k] = _mode_k2_t(S.st["z"], p.dz, p.nzm, p.zlz, PI * k / p.nzm)
+            kim[k], tim[k] = _mode_k2_t(S.st["z"], p.dz, p.nzm, p.zlz, PI * (p.nzm - k) / p.nzm)
+    else:
+        kz2, Tz = _wave_1d(S.st["z"], p.dz, p.nz, p.nzm, p.zlz, 0)
+        kre, tre = kz2[:nzh], Tz[:nzh]
+        kim, tim = kre, tre
+    X2 = kx2[None, None, :]; Y2 = ky2[None, :, None]
+    TX = Tx[None, None, :]; TY = Ty[None, :, None]
+    out = []
+    for kz2, Tz in ((kre, tre), (kim, tim)):
+        Z2 = kz2[:, None, None]; TZ = Tz[:, None, None]
+        out.append(X2 * (TY * TZ) ** 2 + Y2 * (TX * TZ) ** 2 + Z2 * (TX * TY) ** 2)
+    return out[0], out[1]
+
+
+def _reorder(a, axis):
+    """Even/odd reorder of poisson.f90:1037-1095: entry j < n/2 takes point
+    2j, entry j >= n/2 takes point 2n-2j-1 (0-based)."""
+    n = a.shape[axis]
+    idx = np.array([2 * j if j < n // 2 else 2 * n - 2 * j - 1 for j in range(n)])
+    return np.take(a, idx, axis=axis)
+
+
+def _unreorder(a, axis):
+    """Inverse of _reorder (poisson.f90:1414-1460)."""
+    n = a.shape[axis]
+    idx = np.array([2 * j if j < n // 2 else 2 * n - 2 * j - 1 for j in range(n)])
+    out = np.empty_like(a)
+    sl = [slice(None)] * a.ndim
+    for j in range(n):
+        src = [slice(None)] * a.ndim
+        src[axis] = j
+        sl[axis] = idx[j]
+        out[tuple(sl)] = a[tuple(src)]
+    return out
+
+
+def _pair_fwd(t1, t2, A, B, axis):
+    """Forward post-processing of a cosine direction (poisson.f90:1137-1160):
+    entry j >= 1 combined with entry n-j, times 1/2; entry 0 unchanged."""
+    n = t1.shape[axis]
+    j = np.arange(1, n)
+    m = n - j
+    sh = [1] * t1.ndim
+    sh[axis] = n - 1
+    Bv = B[j].reshape(sh); Av = A[j].reshape(sh)
+    tmp1, tmp2 = np.take(t1, j, axis=axis), np.take(t2, j, axis=axis)
+    tmp3, tmp4 = np.take(t1, m, axis=axis), np.take(t2, m, axis=axis)
+    xx1 = tmp1 * Bv * 0.5; xx2 = tmp1 * Av * 0.5; xx3 = tmp2 * Bv * 0.5; xx4 = tmp2 * Av * 0.5
+    xx5 = tmp3 * Bv * 0.5; xx6 = tmp3 * Av * 0.5; xx7 = tmp4 * Bv * 0.5; xx8 = tmp4 * Av * 0.5
+    o1, o2 = t1.copy(), t2.copy()
+    idx = [slice(None)] * t1.ndim
+    idx[axis] = slice(1, n)
+    o1[tuple(idx)] = xx1 + xx4 + xx5 - xx8
+    o2[tuple(idx)] = -xx2 + xx3 + xx6 + xx7
+    return o1, o2
+
+
+def _pair_bwd(t1, t2, A, B, axis):
+    """Backward post-processing of a cosine direction (poisson.f90:1327-1351)."""
+    n = t1.shape[axis]
+    j = np.arange(1, n)
+    m = n - j
+    sh = [1] * t1.ndim
+    sh[axis] = n - 1
+    Bv = B[j].reshape(sh); Av = A[j].reshape(sh)
+    tmp1, tmp2 = np.take(t1, j, axis=axis), np.take(t2, j, axis=axis)
+    tmp3, tmp4 = np.take(t1, m, axis=axis), np.take(t2, m, axis=axis)
+    xx1 = tmp1 * Bv; xx2 = tmp1 * Av; xx3 = tmp2 * Bv; xx4 = tmp2 * Av
+    xx5 = tmp3 * Bv; xx6 = tmp3 * Av; xx7 = tmp4 * Bv; xx8 = tmp4 * Av
+    o1, o2 = t1.copy(), t2.copy()
+    idx = [slice(None)] * t1.ndim
+    idx[axis] = slice(1, n)
+    o1[tuple(idx)] = xx1 - xx4 + xx6 + xx7
+    o2[tuple(idx)] = -(-xx2 - xx3 + xx5 - xx8)
+    return o1, o2
+
+
+def poisson_11x(S: Schemes, rhs):
+    """Poisson with non-periodic x and y (poisson.f90:1019-1465, istret=0),
+    z non-periodic (bcz=1) or periodic, restated step by step on the pressure
+    grid (nz', nym, nxm): even/odd reorders, r2c along z, post-processing in
+    z, y, x, divide real and imaginary parts by their own K, inverse."""
+    p = S.p
+    bcz = 0 if p.nclz else 1
+    nx, ny = p.nxm, p.nym
+    nz = p.nzm if bcz else p.nz
+    assert rhs.shape == (nz, ny, nx)
+    r = _reorder(rhs, 0) if bcz else rhs
+    r = _reorder(r, 1)
+    r = _reorder(r, 2)
+    cw = np.fft.rfft(r, axis=0)
+    cw = np.fft.fft(cw, axis=1)
+    cw = np.fft.fft(cw, axis=2)
+    cw = cw / float(nx) / float(ny) / float(nz)
+    nzh = nz // 2 + 1
+    ax_, bx_ = abxyz(nx, 1)
+    ay_, by_ = abxyz(ny, 1)
+    az_, bz_ = abxyz(nz, bcz)
+    AZ = az_[:nzh, None, None]; BZ = bz_[:nzh, None, None]
+    t1, t2 = cw.real, cw.imag
+    t1, t2 = t1 * BZ + t2 * AZ, t2 * BZ - t1 * AZ          # poisson.f90:1112-1121
+    t1, t2 = _pair_fwd(t1, t2, ay_, by_, 1)               # :1129-1152
+    t1, t2 = _pair_fwd(t1, t2, ax_, bx_, 2)               # :1165-1186
+    Kr, Ki = waves_kxyz_11x(S)
+    c1 = np.where(np.abs(Kr) < EPS_POISSON, 0.0, t1 / (-np.where(np.abs(Kr) < EPS_POISSON, 1.0, Kr)))
+    c2 = np.where(np.abs(Ki) < EPS_POISSON, 0.0, t2 / (-np.where(np.abs(Ki) < EPS_POISSON, 1.0, Ki)))
+    t1, t2 = _pair_bwd(c1, c2, ax_, bx_, 2)               # :1306-1325
+    t1, t2 = _pair_bwd(t1, t2, ay_, by_, 1)               # :1333-1351
+    t1, t2 = t1 * BZ - t2 * AZ, t2 * BZ + t1 * AZ          # :1364-1373
+    cw = t1 + 1j * t2
+    cw = np.fft.ifft(cw, axis=2) * nx
+    cw = np.fft.ifft(cw, axis=1) * ny
+    out = np.fft.irfft(cw, n=nz, axis=0) * nz
+    if bcz:
+        out = _unreorder(out, 0)
+    out = _unreorder(out, 1)
+    return _unreorder(out, 2)
+
+
 def poisson(S: Schemes, rhs):
     """Dispatch as decomp_2d_poisson_init (poisson.f90:98-108)."""
     p = S.p
@@ -905,7 +1051,9 @@ def poisson(S: Schemes, rhs):
         return poisson_000(S, rhs)
     if p.nclx and (not p.ncly) and p.nclz:
         return poisson_010(S, rhs)
-    raise NotImplementedError("poisson_100/11x are not restated")
+    if (not p.nclx) and (not p.ncly):
+        return poisson_11x(S, rhs)
+    raise NotImplementedError("poisson_100 is not restated")
 
 
 # ---------------------------------------------------------------------------
@@ -1035,6 +1183,48 @@ def intt_ab3(var, hist, itime, dt):
     return var
 
 
+def rk3_coefficients(dt):
+    """adt, bdt, gdt of the low-storage RK3 (variables.f90:1388-1399), one
+    entry per sub-step."""
+    adt = [(8.0 / 15.0) * dt, (5.0 / 12.0) * dt, (3.0 / 4.0) * dt]
+    bdt = [0.0, (-17.0 / 60.0) * dt, (-5.0 / 12.0) * dt]
+    gdt = [adt[0], adt[1] + bdt[1], adt[2] + bdt[2]]
+    return adt, bdt, gdt
+
+
+def intt_rk3(var, hist, itr, dt):
+    """intt for itimescheme=5 (time_integrators.f90:151-157): sub-step itr
+    (1-based); hist = [f1, f2] with f1 the current right-hand side, updated
+    in place (f2 <- f1); returns the new var."""
+    adt, bdt, gdt = rk3_coefficients(dt)
+    f1, f2 = hist[0], hist[1]
+    if itr == 1:
+        var = gdt[0] * f1 + var
+    else:
+        var = adt[itr - 1] * f1 + bdt[itr - 1] * f2 + var
+    hist[1] = f1
+    return var
+
+
+def pre_correc(p: "Params", ux, uy, uz):
+    """Free-slip faces of pre_correc (navier.f90:600-614, 686-704, 752-770):
+    the normal velocity is set to zero on ncl=1 boundaries."""
+    ux, uy, uz = ux.copy(), uy.copy(), uz.copy()
+    if p.nclx1 == 1:
+        ux[:, :, 0] = 0.0
+    if p.nclxn == 1:
+        ux[:, :, -1] = 0.0
+    if p.ncly1 == 1:
+        uy[:, 0, :] = 0.0
+    if p.nclyn == 1:
+        uy[:, -1, :] = 0.0
+    if p.nclz1 == 1:
+        uz[0, :, :] = 0.0
+    if p.nclzn == 1:
+        uz[-1, :, :] = 0.0
+    return ux, uy, uz
+
+
 # ---------------------------------------------------------------------------
 # Case-TGV.f90
 # ---------------------------------------------------------------------------
@@ -1096,6 +1286,27 @@ def tgv_state(p: Params):
     ux, uy, uz = init_tgv(p)
     # case.f90:131-136: RHS history seeded with u
     return State(ux, uy, uz, [ux.copy()] * 3, [uy.copy()] * 3, [uz.copy()] * 3, 0)
+
+
+def step_rk3(S: Schemes, st: State):
+    """One RK3 time step (itimescheme=5, xcompact3d.f90:46-87): three
+    sub-steps of calculate_transeq_rhs -> int_time -> pre_correc ->
+    solve_poisson -> cor_vel.  st.hx/hy/hz hold [f1, f2]."""
+    st.itime += 1
+    dt = S.p.dt
+    for itr in (1, 2, 3):
+        dux, duy, duz = momentum_rhs(S, st.ux, st.uy, st.uz)
+        st.hx[0], st.hy[0], st.hz[0] = dux, duy, duz
+        st.ux = intt_rk3(st.ux, st.hx, itr, dt)
+        st.uy = intt_rk3(st.uy, st.hy, itr, dt)
+        st.uz = intt_rk3(st.uz, st.hz, itr, dt)
+        st.ux, st.uy, st.uz = pre_correc(S.p, st.ux, st.uy, st.uz)
+        pp3 = poisson(S, divergence(S, st.ux, st.uy, st.uz))
+        px, py, pz = gradp(S, pp3)
+        st.ux = st.ux - px
+        st.uy = st.uy - py
+        st.uz = st.uz - pz
+    return st
 
 
 def step(S: Schemes, st: State):

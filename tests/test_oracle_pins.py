@@ -190,6 +190,47 @@ def test_poisson_010_is_dct_solve():
     assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
 
 
+FREESLIP = dict(nclx1=1, nclxn=1, ncly1=1, nclyn=1, nclz1=1, nclzn=1)
+
+
+@pytest.mark.parametrize("zfree", [True, False])
+def test_poisson_11x_is_cosine_solve(zfree):
+    """poisson_11x restated step by step (reorders, r2c along z, pairings,
+    real/imaginary split of K) equals a direct cosine-transform solve with K
+    at modes w = pi m / nm (scipy DCT-II), z free-slip or periodic."""
+    import scipy.fft as sf
+    kw = dict(FREESLIP) if zfree else dict(nclx1=1, nclxn=1, ncly1=1, nclyn=1)
+    p = O.Params(17, 9, 13 if zfree else 12, **kw)
+    S = O.Schemes(p)
+    nz = p.nzm if zfree else p.nz
+    f = np.random.default_rng(5).standard_normal((nz, p.nym, p.nxm))
+    a = O.poisson(S, f)
+    kx2, tx = O._wave_1d(S.st["x"], p.dx, p.nx, p.nxm, p.xlx, 1)
+    ky2, ty = O._wave_1d(S.st["y"], p.dy, p.ny, p.nym, p.yly, 1)
+    kz2, tz = O._wave_1d(S.st["z"], p.dz, p.nz, p.nzm, p.zlz, 1 if zfree else 0)
+    K = (kx2[None, None, :] * (ty[None, :, None] * tz[:, None, None]) ** 2
+         + ky2[None, :, None] * (tx[None, None, :] * tz[:, None, None]) ** 2
+         + kz2[:, None, None] * (tx[None, None, :] * ty[None, :, None]) ** 2)
+    F = sf.dctn(f, type=2, axes=(0, 1, 2)) if zfree else np.fft.fft(sf.dctn(f, type=2, axes=(1, 2)), axis=0)
+    Z = K < 1e-16
+    P = np.where(Z, 0.0, F / (-np.where(Z, 1.0, K)))
+    if zfree:
+        b = sf.idctn(P, type=2, axes=(0, 1, 2))
+    else:
+        b = sf.idctn(np.fft.ifft(P, axis=0), type=2, axes=(1, 2)).real
+    assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
+
+
+def test_poisson_11x_inverts_freeslip_div_grad():
+    """On a free-slip box the staggered operators' parities (navier.f90) make
+    div(grad p) a cosine-diagonal operator: the solve returns p minus its mean."""
+    p = O.Params(17, 9, 13, **FREESLIP)
+    S = O.Schemes(p)
+    pe = np.random.default_rng(7).standard_normal((p.nzm, p.nym, p.nxm))
+    pr = O.poisson(S, O.divergence(S, *O.gradp(S, pe)))
+    assert np.abs(pr - (pe - pe.mean())).max() <= 1e-13 * np.abs(pe).max()
+
+
 # ------------------------------------------------------------------ time integration and TGV physics
 def test_ab3_is_exact_for_quadratic_rhs():
     dt = 0.1
@@ -244,3 +285,25 @@ def test_tgv_32_to_t1_matches_reference_dns():
     assert abs(eps - row[3]) / row[3] < 1e-5
     dmax, dmean = O.div_stats(O.divergence(S, st.ux, st.uy, st.uz))
     assert dmax < 1e-12
+
+
+def test_freeslip_rk3_matches_reference_ctest():
+    """The reference's own CTest (tests/TGV-Taylor-Green-vortex): TGV on a
+    65^3 free-slip box of side pi, Re=1600, RK3 (itimescheme=5), dt=0.005,
+    statistics every 10 steps written at (itime-1)*dt.  The first two rows
+    of its reference_time_evol.dat (committed as
+    tests/golden/tgv_freeslip_rk3_time_evol.dat) are matched at the CTest's
+    tolerance (numpy assert_allclose default, rtol 1e-7)."""
+    ref = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "tgv_freeslip_rk3_time_evol.dat"))
+    p = O.Params(65, 65, 65, xlx=math.pi, yly=math.pi, zlz=math.pi, re=1600.0, dt=0.005, itimescheme=5,
+                 **FREESLIP)
+    S = O.Schemes(p)
+    ux, uy, uz = O.init_tgv(p)
+    st = O.State(ux, uy, uz, [ux.copy(), ux.copy()], [uy.copy(), uy.copy()], [uz.copy(), uz.copy()], 0)
+    for it in range(1, 21):
+        O.step_rk3(S, st)
+        if it % 10 == 0:
+            row = ref[it // 10 - 1]
+            assert abs((it - 1) * p.dt - row[0]) < 1e-12
+            np.testing.assert_allclose(O.tgv_diagnostics(S, st.ux, st.uy, st.uz), row[1:], rtol=1e-7)
+
