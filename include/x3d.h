@@ -240,7 +240,13 @@ int x3d_host_poisson(double *rhs, const int *n1, const int *n2, const int *n3);
  * momentum right-hand side and AB3 run as nine fused sweeps and the
  * divergence / gradient + cor_vel as six; other grids (or the environment
  * variable X3D_UNFUSED_MOMENTUM set at create time) use one launch per
- * derivative.  Both paths compute the same quantities. */
+ * derivative.  Both paths compute the same quantities.
+ * Boundaries: periodic or free-slip (ncl=1 at both ends) per direction, with
+ * the Poisson solver of the grid (poisson_000, poisson_010, poisson_11x) and
+ * pre_correc's zero normal velocity on free-slip faces (navier.f90:600-770).
+ * Time schemes: AB3 (itimescheme=3) and the low-storage RK3 (itimescheme=5,
+ * time_integrators.f90:151-157, three projected sub-steps per step); the
+ * fused sweeps run AB3 on periodic boxes. */
 typedef struct x3d_solver x3d_solver;
 int x3d_solver_create(int precision, x3d_solver **out);
 int x3d_solver_destroy(x3d_solver *s);

@@ -474,6 +474,23 @@ __global__ void __launch_bounds__(FFT_THREADS) strided_kernel(cplx<T> *__restric
     // divide by -K in FP64 (poisson.f90:362-371 / :795-819 / :1191-1207).
     // With a cosine transform along x the real part is mode kx and the
     // imaginary part mode nx-kx, each divided by its own K.
+    cplx<T> part[E];
+    if (COS && !dct_here) {
+      // periodic z after the x cosine transform: the z FFT has mixed the
+      // real (mode kx) and imaginary (mode nx-kx) parts; they are the
+      // Hermitian and anti-Hermitian halves in kz, A_k = (D_k + conj D_{N-k})/2
+      // and iB_k = (D_k - conj D_{N-k})/2, each divided by its own K below
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < E; ++m) ld.store(t + m * NT, x[m]);
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + m * NT;
+        part[m] = cconj(ld.load(k == 0 ? 0 : N - k));
+      }
+      __syncthreads();
+    }
     if (valid && COS) {
       const double ky2 = tb.ky2[b], tyv = tb.ty[b];
       const int am = 2 * (nxh - 1) - a;
@@ -487,7 +504,13 @@ __global__ void __launch_bounds__(FFT_THREADS) strided_kernel(cplx<T> *__restric
         const double Ki = kx2i * yz2 + ky2 * (txi * tzv) * (txi * tzv) + kz2 * (txi * tyv) * (txi * tyv);
         const double fr = Kr < 1.0e-16 ? 0.0 : -scale / Kr;
         const double fi = Ki < 1.0e-16 ? 0.0 : -scale / Ki;
-        x[m] = cplx<T>{(T)((double)x[m].x * fr), (T)((double)x[m].y * fi)};
+        if (dct_here) {
+          x[m] = cplx<T>{(T)((double)x[m].x * fr), (T)((double)x[m].y * fi)};
+        } else {
+          const double ar = 0.5 * ((double)x[m].x + (double)part[m].x), ai = 0.5 * ((double)x[m].y + (double)part[m].y);
+          const double br = 0.5 * ((double)x[m].x - (double)part[m].x), bi = 0.5 * ((double)x[m].y - (double)part[m].y);
+          x[m] = cplx<T>{(T)(ar * fr + br * fi), (T)(ai * fr + bi * fi)};
+        }
       }
     } else if (valid) {
       const double ky2 = tb.ky2[b], tyv = tb.ty[b];

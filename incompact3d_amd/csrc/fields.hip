@@ -217,12 +217,19 @@ __global__ void __launch_bounds__(EW_THREADS) tgv_partials(const T *__restrict__
                                                            const T *g2, const T *g3, const T *g4, const T *g5,
                                                            const T *g6, const T *g7, const T *g8, const T *lx,
                                                            const T *ly, const T *lz, double xnu, size_t n,
+                                                           int nx, int ny, int nxc, int nyc, int nzc,
                                                            double *__restrict__ partial) {
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   const size_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const size_t beg = (size_t)blockIdx.x * chunk;
   const size_t end = beg + chunk < n ? beg + chunk : n;
+  const bool trim = nxc < nx || nyc < ny || (size_t)nzc * nx * ny < n;
   for (size_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
+    if (trim) {
+      // free-slip directions drop their last plane (Case-TGV.f90:213-242)
+      const size_t ij = i % ((size_t)nx * ny);
+      if ((int)(ij % nx) >= nxc || (int)(ij / nx) >= nyc || (int)(i / ((size_t)nx * ny)) >= nzc) continue;
+    }
     const double a = ux[i], b = uy[i], c = uz[i];
     s0 += 0.5 * (a * a + b * b + c * c);
     const double ta1 = g0[i], tb1 = g1[i], tc1 = g2[i], td1 = g3[i], te1 = g4[i], tf1 = g5[i], tg1 = g6[i],
@@ -266,18 +273,46 @@ __global__ void __launch_bounds__(EW_THREADS) sum_partials(const double *__restr
 
 template <typename T>
 int launch_tgv_sums(const T *ux, const T *uy, const T *uz, const T *const g[9], const T *lx, const T *ly,
-                    const T *lz, double xnu, size_t n, double *partial, double *out4, hipStream_t st) {
+                    const T *lz, double xnu, int nx, int ny, int nz, int nxc, int nyc, int nzc, double *partial,
+                    double *out4, hipStream_t st) {
+  const size_t n = (size_t)nx * ny * nz;
   hipLaunchKernelGGL((tgv_partials<T>), dim3(RED_BLOCKS), dim3(EW_THREADS), 0, st, ux, uy, uz, g[0], g[1], g[2], g[3],
-                     g[4], g[5], g[6], g[7], g[8], lx, ly, lz, xnu, n, partial);
+                     g[4], g[5], g[6], g[7], g[8], lx, ly, lz, xnu, n, nx, ny, nxc, nyc, nzc, partial);
   hipLaunchKernelGGL(sum_partials, dim3(1), dim3(EW_THREADS), 0, st, partial, RED_BLOCKS, 4, out4);
   X3D_HIP(hipGetLastError());
   return X3D_OK;
 }
 template int launch_tgv_sums<double>(const double *, const double *, const double *, const double *const[9],
-                                     const double *, const double *, const double *, double, size_t, double *,
-                                     double *, hipStream_t);
+                                     const double *, const double *, const double *, double, int, int, int, int, int,
+                                     int, double *, double *, hipStream_t);
 template int launch_tgv_sums<float>(const float *, const float *, const float *, const float *const[9], const float *,
-                                    const float *, const float *, double, size_t, double *, double *, hipStream_t);
+                                    const float *, const float *, double, int, int, int, int, int, int, double *,
+                                    double *, hipStream_t);
+
+// pre_correc on free-slip faces (navier.f90:600-614, 686-704, 752-770): the
+// normal velocity component on plane `idx` of direction `dir` set to zero,
+// in the FP64 velocity and its FP32 copy when there is one.
+__global__ void __launch_bounds__(EW_THREADS) zero_face_kernel(double *__restrict__ u, float *__restrict__ u32,
+                                                               int nx, int ny, int nz, int dir, int idx) {
+  const int na = dir == 0 ? ny : nx, nb = dir == 2 ? ny : nz;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= na * nb) return;
+  const int a = q % na, b = q / na;
+  size_t o;
+  if (dir == 0) o = (size_t)idx + (size_t)nx * (a + (size_t)ny * b);
+  else if (dir == 1) o = (size_t)a + (size_t)nx * (idx + (size_t)ny * b);
+  else o = (size_t)a + (size_t)nx * (b + (size_t)ny * idx);
+  u[o] = 0.0;
+  if (u32) u32[o] = 0.0f;
+}
+
+int launch_zero_face(double *u, float *u32, int nx, int ny, int nz, int dir, int idx, hipStream_t st) {
+  const int cnt = dir == 0 ? ny * nz : (dir == 1 ? nx * nz : nx * ny);
+  hipLaunchKernelGGL(zero_face_kernel, dim3((cnt + EW_THREADS - 1) / EW_THREADS), dim3(EW_THREADS), 0, st, u, u32, nx,
+                     ny, nz, dir, idx);
+  X3D_HIP(hipGetLastError());
+  return X3D_OK;
+}
 
 // max and mean |.| of a field (navier.f90:346-361)
 template <typename T>

@@ -1,6 +1,7 @@
-// Device-resident time step of the incompressible solver (periodic box):
+// Device-resident time step of the incompressible solver (periodic and free-slip boxes):
 //   calculate_transeq_rhs  transeq.f90:20-591   (skew-symmetric convection + nu lap)
-//   int_time (AB3)         time_integrators.f90:85-100
+//   int_time (AB3 / RK3)   time_integrators.f90:85-100, 151-157
+//   pre_correc             navier.f90:502-789   (free-slip faces)
 //   solve_poisson          navier.f90:27-141    (divergence, poisson, gradp)
 //   cor_vel                navier.f90:206-249
 // as in the time loop of xcompact3d.f90:29-102.  On grids the fused sweeps
@@ -11,7 +12,8 @@
 // does the reference's array arithmetic.  precision 0 computes in FP64;
 // precision 1 keeps an FP32 copy of the velocity for every derivative, FFT
 // and product, while the velocity itself is accumulated in FP64 (AB3 and
-// cor_vel) and the Poisson divide and reductions run in FP64.
+// cor_vel) and the Poisson divide and reductions run in FP64.  RK3
+// (itimescheme=5) and free-slip boundaries run on the per-operator path.
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -32,7 +34,9 @@ int launch_tgv(double *ux, double *uy, double *uz, int nx, int ny, int nz, doubl
                hipStream_t st);
 template <typename T>
 int launch_tgv_sums(const T *ux, const T *uy, const T *uz, const T *const g[9], const T *lx, const T *ly,
-                    const T *lz, double xnu, size_t n, double *partial, double *out4, hipStream_t st);
+                    const T *lz, double xnu, int nx, int ny, int nz, int nxc, int nyc, int nzc, double *partial,
+                    double *out4, hipStream_t st);
+int launch_zero_face(double *u, float *u32, int nx, int ny, int nz, int dir, int idx, hipStream_t st);
 template <typename T>
 int launch_max_mean(const T *a, size_t n, double *partial, double *out2, hipStream_t st);
 
@@ -240,6 +244,62 @@ int momentum_yz(Context &c, x3d_solver *S, int nrep, double *bytes) {
   return X3D_OK;
 }
 
+// pre_correc (navier.f90:502-789) for the boundaries this solver covers:
+// zero normal velocity on free-slip faces (nothing for periodic ones)
+int pre_correc(const Context &c, x3d_solver *S) {
+  float *const u32[3] = {S->u32[0], S->u32[1], S->u32[2]};
+  const int nn[3] = {S->nx, S->ny, S->nz};
+  for (int d = 0; d < 3; ++d) {
+    if (c.dir[d].bc1 == 1) X3D_TRY(launch_zero_face(S->u64[d], u32[d], S->nx, S->ny, S->nz, d, 0, c.stream));
+    if (c.dir[d].bcn == 1)
+      X3D_TRY(launch_zero_face(S->u64[d], u32[d], S->nx, S->ny, S->nz, d, nn[d] - 1, c.stream));
+  }
+  return X3D_OK;
+}
+
+// One RK3 step (itimescheme=5, xcompact3d.f90:46-87): three sub-steps of
+// transeq RHS, the low-storage update of time_integrators.f90:151-157 with
+// the coefficients of variables.f90:1388-1399 (FP64 accumulation),
+// pre_correc, divergence, Poisson, gradp + cor_vel.  hist slot[0] holds f1,
+// slot[1] the previous sub-step's f1.
+template <typename T>
+int step_rk3(Context &c, x3d_solver *S) {
+  const int nx = S->nx, ny = S->ny, nz = S->nz;
+  T *u[3];
+  for (int i = 0; i < 3; ++i) u[i] = sizeof(T) == 8 ? (T *)S->u64[i] : (T *)S->u32[i];
+  T *W[4] = {(T *)S->w[0], (T *)S->w[1], (T *)S->w[2], (T *)S->w[3]};
+  S->itime += 1;
+  const double dt = c.p.dt;
+  const double adt[3] = {(8.0 / 15.0) * dt, (5.0 / 12.0) * dt, (3.0 / 4.0) * dt};
+  const double bdt[3] = {0.0, (-17.0 / 60.0) * dt, (-5.0 / 12.0) * dt};
+  const T *cu[3] = {u[0], u[1], u[2]};
+  for (int itr = 0; itr < 3; ++itr) {
+    T *sgs[3] = {nullptr, nullptr, nullptr};
+    if (S->les) {
+      for (int i = 0; i < 3; ++i) sgs[i] = (T *)S->lesw + (17 + i) * S->n;
+      X3D_TRY(compute_sgs<T>(c, S, cu, sgs));
+    }
+    for (int comp = 0; comp < 3; ++comp) {
+      T *f1 = (T *)S->hist[comp][S->slot[0]];
+      X3D_TRY(momentum_component<T>(c, cu, comp, W[0], f1, nx, ny, nz));
+      if (S->les) X3D_TRY(launch_add<T>(f1, sgs[comp], S->n, c.stream));
+    }
+    for (int comp = 0; comp < 3; ++comp) {
+      const T *f1 = (T *)S->hist[comp][S->slot[0]];
+      const T *f2 = (T *)S->hist[comp][S->slot[1]];
+      // itr 1: u += gdt(1) f1 with gdt(1) = adt(1); then u += adt f1 + bdt f2
+      X3D_TRY(launch_ab<T>(S->u64[comp], f1, f2, nullptr, adt[itr], bdt[itr], 0.0, itr == 0 ? 1 : 2,
+                           sizeof(T) == 8 ? nullptr : S->u32[comp], S->n, c.stream));
+    }
+    std::swap(S->slot[0], S->slot[1]);   // f2 <- f1
+    X3D_TRY(pre_correc(c, S));
+    X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));
+    X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
+    X3D_TRY(gradp_correct<T>(c, S, u, W));
+  }
+  return X3D_OK;
+}
+
 template <typename T>
 int step_once(Context &c, x3d_solver *S) {
   const int nx = S->nx, ny = S->ny, nz = S->nz;
@@ -330,6 +390,7 @@ int step_once(Context &c, x3d_solver *S) {
     X3D_TRY(launch_projection<T>(c, SW_GRAD_X, gx_in, onone, u64, sizeof(T) == 8 ? nullptr : u32, nx, ny, nz,
                                  c.stream));
   } else {
+    X3D_TRY(pre_correc(c, S));
     X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));
     X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
     X3D_TRY(gradp_correct<T>(c, S, u, W));
@@ -363,11 +424,15 @@ int tgv_stats(Context &c, x3d_solver *S, double out4[4]) {
   for (int i = 0; i < 9; ++i) gg[i] = g[i];
   if (!S->partial) X3D_HIP(hipMalloc(&S->partial, 8 * 1024 * sizeof(double)));
   if (!S->red) X3D_HIP(hipMalloc(&S->red, 8 * sizeof(double)));
-  X3D_TRY(launch_tgv_sums<T>(u[0], u[1], u[2], gg, g[9], g[10], g[11], c.xnu, n, S->partial, S->red, c.stream));
+  // free-slip directions count nm = n-1 cells (Case-TGV.f90:213-242)
+  const int nxc = c.dir[0].bc1 == 1 ? c.nxm : nx, nyc = c.dir[1].bc1 == 1 ? c.nym : ny,
+            nzc = c.dir[2].bc1 == 1 ? c.nzm : nz;
+  X3D_TRY(launch_tgv_sums<T>(u[0], u[1], u[2], gg, g[9], g[10], g[11], c.xnu, nx, ny, nz, nxc, nyc, nzc, S->partial,
+                             S->red, c.stream));
   double h[4];
   X3D_HIP(hipMemcpyAsync(h, S->red, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   X3D_HIP(hipStreamSynchronize(c.stream));
-  const double ncell = (double)nx * ny * nz;
+  const double ncell = (double)nxc * nyc * nzc;
   out4[0] = h[0] / ncell;   // eek
   out4[1] = h[1] / ncell;   // eps
   out4[2] = h[3] / ncell;   // eps2
@@ -445,17 +510,22 @@ int x3d_solver_create(int precision, x3d_solver **out) {
   X3D_TRY(require_ctx(&c));
   if (!out) return fail(X3D_ERR_ARG, "null output");
   if (precision != 0 && precision != 1) return fail(X3D_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 bulk)");
-  if (c->dir[0].bc1 != 0 || c->dir[1].bc1 != 0 || c->dir[2].bc1 != 0)
-    return fail(X3D_ERR_UNSUPPORTED, "the device time step covers triply periodic boxes");
+  for (int d = 0; d < 3; ++d) {
+    const int b1 = c->dir[d].bc1, bn = c->dir[d].bcn;
+    if (!((b1 == 0 && bn == 0) || (b1 == 1 && bn == 1)))
+      return fail(X3D_ERR_UNSUPPORTED, "the device time step covers periodic and free-slip (ncl=1) boundaries");
+  }
   if (!c->poisson) return fail(X3D_ERR_UNSUPPORTED, "no Poisson solver for this grid");
-  if (c->p.itimescheme != 3) return fail(X3D_ERR_UNSUPPORTED, "the device time step implements AB3 (itimescheme=3)");
+  if (c->p.itimescheme != 3 && c->p.itimescheme != 5)
+    return fail(X3D_ERR_UNSUPPORTED, "the device time step implements AB3 (itimescheme=3) and RK3 (itimescheme=5)");
   auto *S = new x3d_solver();
   S->precision = precision;
   S->nx = c->p.nx;
   S->ny = c->p.ny;
   S->nz = c->p.nz;
   S->n = (size_t)S->nx * S->ny * S->nz;
-  S->fused = momentum_fusable(*c) && projection_fusable(*c) && getenv("X3D_UNFUSED_MOMENTUM") == nullptr;
+  S->fused = c->p.itimescheme == 3 && momentum_fusable(*c) && projection_fusable(*c) &&
+             getenv("X3D_UNFUSED_MOMENTUM") == nullptr;
   int r = alloc_solver(*c, S);
   if (r != X3D_OK) {
     free_solver(S);
@@ -509,8 +579,14 @@ int x3d_solver_step(x3d_solver *S, int nsteps) {
   X3D_TRY(require_ctx(&c));
   if (!S || nsteps < 0) return fail(X3D_ERR_ARG, "bad arguments");
   for (int k = 0; k < nsteps; ++k) {
-    if (S->precision == 0) X3D_TRY(step_once<double>(*c, S));
-    else X3D_TRY(step_once<float>(*c, S));
+    if (c->p.itimescheme == 5) {
+      if (S->precision == 0) X3D_TRY(step_rk3<double>(*c, S));
+      else X3D_TRY(step_rk3<float>(*c, S));
+    } else if (S->precision == 0) {
+      X3D_TRY(step_once<double>(*c, S));
+    } else {
+      X3D_TRY(step_once<float>(*c, S));
+    }
   }
   return X3D_OK;
 }

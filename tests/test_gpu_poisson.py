@@ -1,4 +1,5 @@
-"""GPU parity of the pressure Poisson solver (poisson_000 / poisson_010) and
+"""GPU parity of the pressure Poisson solver (poisson_000 / poisson_010 /
+poisson_11x with z free-slip or periodic) and
 of the pencil transposes against the oracle, through the C-ABI.
 
 Bar: 1e-12 relative (FP64); the FP32 entry (FP32 FFTs, FP64 divide) is
@@ -19,6 +20,9 @@ CASES = [
     ("p000_128", dict(nx=128, ny=128, nz=128)),
     ("p010_32x33x16", dict(nx=32, ny=33, nz=16, ncly1=2, nclyn=2)),
     ("p010_64x17x32", dict(nx=64, ny=17, nz=32, ncly1=1, nclyn=1)),
+    ("p11x_bcz1_33x17x65", dict(nx=33, ny=17, nz=65, nclx1=1, nclxn=1, ncly1=1, nclyn=1, nclz1=1, nclzn=1)),
+    ("p11x_bcz0_65x33x32", dict(nx=65, ny=33, nz=32, nclx1=1, nclxn=1, ncly1=1, nclyn=1)),
+    ("p11x_bcz1_65", dict(nx=65, ny=65, nz=65, nclx1=1, nclxn=1, ncly1=1, nclyn=1, nclz1=1, nclzn=1)),
 ]
 
 
