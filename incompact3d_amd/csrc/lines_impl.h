@@ -565,7 +565,7 @@ __device__ __forceinline__ void wave_solve(T (&v)[V], const T (&fs)[V], const T 
 // within 64 KiB.
 template <typename T>
 struct YzShape {
-  static constexpr int L = sizeof(T) == 8 ? 8 : 16;
+  static constexpr int L = sizeof(T) == 8 ? SEG_L64 : SEG_L32;
   int P = 0, TL = 0;
   size_t shmem = 0;
   bool plan(int n, int nmat, int nrhs) {

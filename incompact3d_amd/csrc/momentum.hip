@@ -13,7 +13,10 @@
 // one set of block barriers.  The x sweep, the last one in the
 // reference's order, also applies AB3 (time_integrators.f90:85-100): the new
 // velocity goes to a second buffer, because the other components still read
-// the old one.
+// the old one.  With FP32 compute the FP64 velocity is not touched here: the
+// sweep writes the step's increment in FP32 and the new FP32 velocity, and the
+// gradient correction adds the increment to the FP64 velocity, so that the
+// FP64 state is read and written once per step.
 #include "sweeps.h"
 
 namespace x3d {
@@ -30,11 +33,13 @@ struct MomArgs {
   int n, na, nb;
   long long sa, sb, se;         // line strides (y/z sweeps)
   int kdepth;                   // carry-chain depth in segments (y/z sweeps)
-  // AB3 epilogue (x sweep only): u_new = ca*f1 + cb*f2 + cc*f3 + u_old
+  // AB3 epilogue (x sweep only), du = ca*f1 + cb*f2 + cc*f3 in FP64:
+  //   FP64: u_new = du + u_c
+  //   FP32: delta = du, shadow = du + u_c (the FP64 velocity adds delta later)
   const T *f2, *f3;
-  const double *u_old;
   double *u_new;
-  float *shadow;                // FP32 copy of u_new (FP32-bulk path)
+  float *delta;
+  float *shadow;
   double ca, cb, cc;
   int nterms;                   // 0: no AB3
 };
@@ -169,9 +174,7 @@ __global__ void __launch_bounds__(256) mom_x_kernel(const MomArgs<T> A, int nlin
     // every epilogue input is read before the first store (the compiler
     // cannot move loads above stores to possibly aliasing arrays)
     T acc[V], f2[V], f3[V];
-    double uo[V];
     if (A.acc) load_run<T, V>(A.acc + base, acc);
-    if (A.nterms) load_run<double, V>(A.u_old + base, uo);
     if (A.nterms >= 2) load_run<T, V>(A.f2 + base, f2);
     if (A.nterms >= 3) load_run<T, V>(A.f3 + base, f3);
     if (A.acc) {
@@ -180,19 +183,28 @@ __global__ void __launch_bounds__(256) mom_x_kernel(const MomArgs<T> A, int nlin
     }
     store_run<T, V>(A.out + base, r);
     if (A.nterms) {
-      // AB3 with FP64 accumulation: u_new = a f1 + b f2 + c f3 + u_old
-      double un[V];
+      // AB3 with FP64 accumulation: du = a f1 + b f2 + c f3, u_new = du + u_old
+      double du[V];
 #pragma unroll
       for (int m = 0; m < V; ++m) {
-        if (A.nterms == 1) un[m] = A.ca * (double)r[m] + uo[m];
-        else if (A.nterms == 2) un[m] = A.ca * (double)r[m] + A.cb * (double)f2[m] + uo[m];
-        else un[m] = A.ca * (double)r[m] + A.cb * (double)f2[m] + A.cc * (double)f3[m] + uo[m];
+        if (A.nterms == 1) du[m] = A.ca * (double)r[m];
+        else if (A.nterms == 2) du[m] = A.ca * (double)r[m] + A.cb * (double)f2[m];
+        else du[m] = A.ca * (double)r[m] + A.cb * (double)f2[m] + A.cc * (double)f3[m];
       }
-      store_run<double, V>(A.u_new + base, un);
-      if (A.shadow) {
-        float sh[V];
+      if constexpr (sizeof(T) == 8) {
+        // u_c is the FP64 velocity itself
+        double un[V];
 #pragma unroll
-        for (int m = 0; m < V; ++m) sh[m] = (float)un[m];
+        for (int m = 0; m < V; ++m) un[m] = du[m] + (double)uc[m];
+        store_run<double, V>(A.u_new + base, un);
+      } else {
+        float d[V], sh[V];
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+          d[m] = (float)du[m];
+          sh[m] = (float)(du[m] + (double)uc[m]);
+        }
+        store_run<float, V>(A.delta + base, d);
         store_run<float, V>(A.shadow + base, sh);
       }
     }
@@ -226,7 +238,7 @@ bool momentum_fusable(const Context &c) {
 
 template <typename T>
 int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], const T *acc, T *out,
-                    const T *f2, const T *f3, const double *u_old, double *u_new, float *shadow, double ca,
+                    const T *f2, const T *f3, double *u_new, float *delta, float *shadow, double ca,
                     double cb, double cc, int nterms, int nx, int ny, int nz, hipStream_t st) {
   const DirCoef &D = c.dir[dir];
   if (D.bc1 != 0) return fail(X3D_ERR_UNSUPPORTED, "fused momentum sweep: periodic lines only");
@@ -249,13 +261,15 @@ int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], co
   A.s = cast_sten<T>(D.s);
   A.xnu = (T)c.xnu;
   A.n = D.n;
-  A.f2 = f2; A.f3 = f3; A.u_old = u_old; A.u_new = u_new; A.shadow = shadow;
+  A.f2 = f2; A.f3 = f3; A.u_new = u_new; A.delta = delta; A.shadow = shadow;
   A.ca = ca; A.cb = cb; A.cc = cc; A.nterms = nterms;
   const int n = D.n;
   if (dir == 0) {
     const int V = n / 64;
     if (n % 64 != 0 || !(V == 1 || V == 2 || V == 4 || V == 8))
       return fail(X3D_ERR_UNSUPPORTED, "fused x sweep needs nx = 64, 128, 256 or 512");
+    if (nterms && (sizeof(T) == 8 ? !u_new : (!delta || !shadow)))
+      return fail(X3D_ERR_ARG, "momentum: AB3 output buffers missing");
     const int nlines = ny * nz;
     const int waves = nlines < 16384 ? nlines : 16384;
     const dim3 grid((waves + 3) / 4), block(256);
@@ -288,10 +302,10 @@ int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], co
 }
 
 template int launch_momentum<double>(const Context &, int, int, const double *const[3], const double *, double *,
-                                     const double *, const double *, const double *, double *, float *, double,
-                                     double, double, int, int, int, int, hipStream_t);
+                                     const double *, const double *, double *, float *, float *, double, double,
+                                     double, int, int, int, int, hipStream_t);
 template int launch_momentum<float>(const Context &, int, int, const float *const[3], const float *, float *,
-                                    const float *, const float *, const double *, double *, float *, double, double,
+                                    const float *, const float *, double *, float *, float *, double, double,
                                     double, int, int, int, int, hipStream_t);
 
 }  // namespace x3d

@@ -71,6 +71,7 @@ struct ProjArgs {
   int kdepth;                   // carry-chain depth in segments (y/z sweeps)
   double *u64[3];               // GRAD_X: the FP64 velocity, corrected in place
   float *u32[3];                // GRAD_X, FP32 bulk: its FP32 copy (else null)
+  const float *du[3];           // GRAD_X, FP32 bulk: the AB3 increment (else null)
 };
 
 // ---------------------------------------------------------------- y / z sweeps
@@ -148,15 +149,22 @@ __global__ void __launch_bounds__(1024) proj_yz_kernel(const ProjArgs<T> A) {
 
 // ---------------------------------------------------------------- x sweeps (one wavefront per line)
 // cor_vel (navier.f90:206-249) on one run of V points: u -= g in FP64, and
-// the FP32 copy refreshed for FP32-bulk compute
+// for FP32-bulk compute first u += the step's AB3 increment, then the FP32
+// copy refreshed
 template <int V, typename T>
 __device__ __forceinline__ void correct(const ProjArgs<T> &A, const int c, const long long base, const T (&g)[V]) {
   double u[V];
   load_run<double, V>(A.u64[c] + base, u);
+  if (sizeof(T) == 4 && A.du[c]) {
+    float d[V];
+    load_run<float, V>(A.du[c] + base, d);
+#pragma unroll
+    for (int m = 0; m < V; ++m) u[m] = u[m] + (double)d[m];
+  }
 #pragma unroll
   for (int m = 0; m < V; ++m) u[m] = u[m] - (double)g[m];
   store_run<double, V>(A.u64[c] + base, u);
-  if (A.u32[c]) {
+  if (sizeof(T) == 4 && A.u32[c]) {
     float f[V];
 #pragma unroll
     for (int m = 0; m < V; ++m) f[m] = (float)u[m];
@@ -257,7 +265,7 @@ static void launch_x(const ProjArgs<T> &A, int V, int nlines, hipStream_t st) {
 // by SW_GRAD_X only (u32 null for FP64 compute).
 template <typename T>
 int launch_projection(const Context &c, int sw, const T *const in[3], T *const out[3], double *const u64[3],
-                      float *const u32[3], int nx, int ny, int nz, hipStream_t st) {
+                      float *const u32[3], const float *const du[3], int nx, int ny, int nz, hipStream_t st) {
   if (sw < SW_DIV_X || sw > SW_GRAD_X) return fail(X3D_ERR_ARG, "projection: unknown sweep");
   const int dir = (sw == SW_DIV_X || sw == SW_GRAD_X) ? 0 : (sw == SW_DIV_Y || sw == SW_GRAD_Y) ? 1 : 2;
   const bool pv = sw >= SW_GRAD_Z;
@@ -271,6 +279,7 @@ int launch_projection(const Context &c, int sw, const T *const in[3], T *const o
     A.out[i] = out ? out[i] : nullptr;
     A.u64[i] = u64 ? u64[i] : nullptr;
     A.u32[i] = u32 ? u32[i] : nullptr;
+    A.du[i] = du ? du[i] : nullptr;
   }
   TriDev<T> tI, tD;
   if constexpr (sizeof(T) == 8) { tI = sI.dev64(); tD = sD.dev64(); }
@@ -314,8 +323,10 @@ int launch_projection(const Context &c, int sw, const T *const in[3], T *const o
 }
 
 template int launch_projection<double>(const Context &, int, const double *const[3], double *const[3],
-                                       double *const[3], float *const[3], int, int, int, hipStream_t);
+                                       double *const[3], float *const[3], const float *const[3], int, int, int,
+                                       hipStream_t);
 template int launch_projection<float>(const Context &, int, const float *const[3], float *const[3],
-                                      double *const[3], float *const[3], int, int, int, hipStream_t);
+                                      double *const[3], float *const[3], const float *const[3], int, int, int,
+                                      hipStream_t);
 
 }  // namespace x3d

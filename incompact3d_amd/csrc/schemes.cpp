@@ -268,7 +268,7 @@ TriDev<double> TriSet::dev64() const {
   TriDev<double> d;
   d.pk = d64;
   d.alfa = alfa; d.sm_den = sm_den; d.rden = 1.0 / sm_den;
-  d.n = (int)ff.size(); d.periodic = periodic; d.kdepth = kd8;
+  d.n = (int)ff.size(); d.periodic = periodic; d.kdepth = kd64;
   return d;
 }
 
@@ -276,7 +276,7 @@ TriDev<float> TriSet::dev32() const {
   TriDev<float> d;
   d.pk = d32;
   d.alfa = (float)alfa; d.sm_den = (float)sm_den; d.rden = (float)(1.0 / sm_den);
-  d.n = (int)ff.size(); d.periodic = periodic; d.kdepth = kd16;
+  d.n = (int)ff.size(); d.periodic = periodic; d.kdepth = kd32;
   return d;
 }
 
@@ -332,10 +332,9 @@ int upload_set(TriSet &s) {
   X3D_HIP(hipMalloc(&s.d32, 4 * n * sizeof(float)));
   X3D_HIP(hipMemcpy(s.d64, h.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
   X3D_HIP(hipMemcpy(s.d32, f.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
-  // carry-chain depths of the segmented y/z solvers (8-point segments in
-  // FP64, 16-point in FP32)
-  s.kd8 = chain_depth(s, 8, 1e-18);
-  s.kd16 = chain_depth(s, 16, 1e-9);
+  // carry-chain depths of the segmented y/z solvers
+  s.kd64 = chain_depth(s, SEG_L64, 1e-18);
+  s.kd32 = chain_depth(s, SEG_L32, 1e-9);
   return X3D_OK;
 }
 

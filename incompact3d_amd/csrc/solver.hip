@@ -336,13 +336,18 @@ int step_once(Context &c, x3d_solver *S) {
                                  0, 0, nx, ny, nz, c.stream));
       X3D_TRY(launch_momentum<T>(c, 1, comp, cu, W[0], W[0], nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0,
                                  0, nx, ny, nz, c.stream));
-      X3D_TRY(launch_momentum<T>(c, 0, comp, cu, W[0], f1, f2, f3, S->u64[comp], S->u64b[comp],
-                                 sizeof(T) == 8 ? nullptr : S->u32b[comp], a, b, cc, nterms, nx, ny, nz,
-                                 c.stream));
+      // FP64: the new velocity into u64b; FP32: the increment into u64b (as
+      // FP32) and the new FP32 velocity into u32b, u64 updated by GRAD_X
+      if constexpr (sizeof(T) == 8)
+        X3D_TRY(launch_momentum<T>(c, 0, comp, cu, W[0], f1, f2, f3, S->u64b[comp], nullptr, nullptr, a, b, cc,
+                                   nterms, nx, ny, nz, c.stream));
+      else
+        X3D_TRY(launch_momentum<T>(c, 0, comp, cu, W[0], f1, f2, f3, nullptr, (float *)S->u64b[comp],
+                                   S->u32b[comp], a, b, cc, nterms, nx, ny, nz, c.stream));
     }
     for (int i = 0; i < 3; ++i) {
-      std::swap(S->u64[i], S->u64b[i]);
-      std::swap(S->u32[i], S->u32b[i]);
+      if (sizeof(T) == 8) std::swap(S->u64[i], S->u64b[i]);
+      else std::swap(S->u32[i], S->u32b[i]);
       u[i] = sizeof(T) == 8 ? (T *)S->u64[i] : (T *)S->u32[i];
       cu[i] = u[i];
     }
@@ -370,25 +375,26 @@ int step_once(Context &c, x3d_solver *S) {
     T *const onone[3] = {nullptr, nullptr, nullptr};
     const T *const dx_in[3] = {cu[0], cu[1], cu[2]};
     T *const dx_out[3] = {W[0], W[1], W[2]};
-    X3D_TRY(launch_projection<T>(c, SW_DIV_X, dx_in, dx_out, nullptr, nullptr, nx, ny, nz, c.stream));
+    X3D_TRY(launch_projection<T>(c, SW_DIV_X, dx_in, dx_out, nullptr, nullptr, nullptr, nx, ny, nz, c.stream));
     const T *const dy_in[3] = {W[0], W[1], W[2]};
     T *const dy_out[3] = {W[3], W[1], nullptr};
-    X3D_TRY(launch_projection<T>(c, SW_DIV_Y, dy_in, dy_out, nullptr, nullptr, nx, ny, nz, c.stream));
+    X3D_TRY(launch_projection<T>(c, SW_DIV_Y, dy_in, dy_out, nullptr, nullptr, nullptr, nx, ny, nz, c.stream));
     const T *const dz_in[3] = {W[3], W[1], nullptr};
     T *const dz_out[3] = {W[0], nullptr, nullptr};
-    X3D_TRY(launch_projection<T>(c, SW_DIV_Z, dz_in, dz_out, nullptr, nullptr, nx, ny, nz, c.stream));
+    X3D_TRY(launch_projection<T>(c, SW_DIV_Z, dz_in, dz_out, nullptr, nullptr, nullptr, nx, ny, nz, c.stream));
     X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
     const T *const gz_in[3] = {W[0], nullptr, nullptr};
     T *const gz_out[3] = {W[1], W[2], nullptr};
-    X3D_TRY(launch_projection<T>(c, SW_GRAD_Z, gz_in, gz_out, nullptr, nullptr, nx, ny, nz, c.stream));
+    X3D_TRY(launch_projection<T>(c, SW_GRAD_Z, gz_in, gz_out, nullptr, nullptr, nullptr, nx, ny, nz, c.stream));
     const T *const gy_in[3] = {W[1], W[2], nullptr};
     T *const gy_out[3] = {W[3], W[0], W[1]};
-    X3D_TRY(launch_projection<T>(c, SW_GRAD_Y, gy_in, gy_out, nullptr, nullptr, nx, ny, nz, c.stream));
+    X3D_TRY(launch_projection<T>(c, SW_GRAD_Y, gy_in, gy_out, nullptr, nullptr, nullptr, nx, ny, nz, c.stream));
     const T *const gx_in[3] = {W[3], W[0], W[1]};
     double *const u64[3] = {S->u64[0], S->u64[1], S->u64[2]};
     float *const u32[3] = {S->u32[0], S->u32[1], S->u32[2]};
-    X3D_TRY(launch_projection<T>(c, SW_GRAD_X, gx_in, onone, u64, sizeof(T) == 8 ? nullptr : u32, nx, ny, nz,
-                                 c.stream));
+    const float *const du[3] = {(const float *)S->u64b[0], (const float *)S->u64b[1], (const float *)S->u64b[2]};
+    X3D_TRY(launch_projection<T>(c, SW_GRAD_X, gx_in, onone, u64, sizeof(T) == 8 ? nullptr : u32,
+                                 sizeof(T) == 8 ? nullptr : du, nx, ny, nz, c.stream));
   } else {
     X3D_TRY(pre_correc(c, S));
     X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));

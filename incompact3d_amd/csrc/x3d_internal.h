@@ -67,10 +67,15 @@ struct TriSet {
   int periodic = 0;
   double *d64 = nullptr;        // packed {fs, fw, ff*fw, r} x n, FP64
   float *d32 = nullptr;         // the same, FP32
-  int kd8 = 0, kd16 = 0;        // chain depths for 8- and 16-point segments
+  int kd64 = 0, kd32 = 0;       // chain depths of the FP64 and FP32 segmented solvers
   TriDev<double> dev64() const;
   TriDev<float> dev32() const;
 };
+
+// Segment lengths of the segmented y/z solvers in FP64 and FP32.
+// (8-point FP32 segments were measured slower: 72 us against 59 us for the
+// fused FP32 momentum sweep at 256^3, from the doubled carry chain.)
+constexpr int SEG_L64 = 8, SEG_L32 = 16;
 
 // Upload the packed device copies of one set (allocates d64/d32).
 int upload_set(TriSet &s);
@@ -161,11 +166,14 @@ int poisson_solve(Context &c, T *rhs, hipStream_t st);
 
 // ---------------------------------------------------------------- fused sweeps
 // momentum.hip: the transeq right-hand side of component comp along dir
-// (out = acc + ...); the x sweep also applies AB3 (nterms > 0) into u_new.
+// (out = acc + ...); the x sweep also applies AB3 (nterms > 0): FP64 compute
+// writes the new velocity to u_new; FP32 compute writes the FP32 increment
+// to delta and the new FP32 velocity to shadow, and the FP64 velocity takes
+// the increment in the gradient correction.
 bool momentum_fusable(const Context &c);
 template <typename T>
 int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], const T *acc, T *out,
-                    const T *f2, const T *f3, const double *u_old, double *u_new, float *shadow, double ca,
+                    const T *f2, const T *f3, double *u_new, float *delta, float *shadow, double ca,
                     double cb, double cc, int nterms, int nx, int ny, int nz, hipStream_t st);
 
 // projection.hip: divergence (x, y, z) and gradient + cor_vel (z, y, x)
@@ -173,7 +181,7 @@ enum { SW_DIV_X = 0, SW_DIV_Y, SW_DIV_Z, SW_GRAD_Z, SW_GRAD_Y, SW_GRAD_X };
 bool projection_fusable(const Context &c);
 template <typename T>
 int launch_projection(const Context &c, int sw, const T *const in[3], T *const out[3], double *const u64[3],
-                      float *const u32[3], int nx, int ny, int nz, hipStream_t st);
+                      float *const u32[3], const float *const du[3], int nx, int ny, int nz, hipStream_t st);
 
 // ---------------------------------------------------------------- closure network (mlp.hip)
 struct MlpDev;
