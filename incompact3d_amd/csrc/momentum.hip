@@ -132,11 +132,22 @@ __global__ void __launch_bounds__(256) mom_x_kernel(const MomArgs<T> A, int nlin
     fs1[m] = A.pk1[4 * e]; fw1[m] = A.pk1[4 * e + 1]; ffw1[m] = A.pk1[4 * e + 2]; sm1[m] = A.pk1[4 * e + 3];
     fs2[m] = A.pk2[4 * e]; fw2[m] = A.pk2[4 * e + 1]; ffw2[m] = A.pk2[4 * e + 2]; sm2[m] = A.pk2[4 * e + 3];
   }
+  // the next line's velocities are loaded before this line is solved, so
+  // that their latency overlaps the solves (each wave loops over several lines)
+  T pud[V], puc[V];
+  if (wave < nlines) {
+    load_run<T, V>(A.ud + (long long)wave * n + e0, pud);
+    load_run<T, V>(A.uc + (long long)wave * n + e0, puc);
+  }
   for (int ln = wave; ln < nlines; ln += nwaves) {
     const long long base = (long long)ln * n + e0;
     T ud[V], uc[V];
-    load_run<T, V>(A.ud + base, ud);
-    load_run<T, V>(A.uc + base, uc);
+#pragma unroll
+    for (int m = 0; m < V; ++m) { ud[m] = pud[m]; uc[m] = puc[m]; }
+    if (ln + nwaves < nlines) {
+      load_run<T, V>(A.ud + (long long)(ln + nwaves) * n + e0, pud);
+      load_run<T, V>(A.uc + (long long)(ln + nwaves) * n + e0, puc);
+    }
     T r[V];
     {
       T p[V];
@@ -271,7 +282,7 @@ int launch_momentum(const Context &c, int dir, int comp, const T *const u[3], co
     if (nterms && (sizeof(T) == 8 ? !u_new : (!delta || !shadow)))
       return fail(X3D_ERR_ARG, "momentum: AB3 output buffers missing");
     const int nlines = ny * nz;
-    const int waves = nlines < 16384 ? nlines : 16384;
+    const int waves = nlines < 8192 ? nlines : 8192;
     const dim3 grid((waves + 3) / 4), block(256);
     switch (V) {
       case 1: hipLaunchKernelGGL((mom_x_kernel<T, 1>), grid, block, 0, st, A, nlines); break;

@@ -189,13 +189,28 @@ __global__ void __launch_bounds__(256) proj_x_kernel(const ProjArgs<T> A, int nl
     fsI[m] = A.pkI[4 * e]; fwI[m] = A.pkI[4 * e + 1]; ffwI[m] = A.pkI[4 * e + 2]; smI[m] = A.pkI[4 * e + 3];
     fsD[m] = A.pkD[4 * e]; fwD[m] = A.pkD[4 * e + 1]; ffwD[m] = A.pkD[4 * e + 2]; smD[m] = A.pkD[4 * e + 3];
   }
+  // the next line's inputs are loaded before this line is solved, so that
+  // their latency overlaps the solves (each wave loops over several lines)
+  T pre[R::NIN][V];
+  if (wave < nlines) {
+#pragma unroll
+    for (int i = 0; i < R::NIN; ++i) load_run<T, V>(A.in[i] + (long long)wave * n + e0, pre[i]);
+  }
   for (int ln = wave; ln < nlines; ln += nwaves) {
     const long long base = (long long)ln * n + e0;
+    T cur[R::NIN][V];
+#pragma unroll
+    for (int i = 0; i < R::NIN; ++i)
+#pragma unroll
+      for (int m = 0; m < V; ++m) cur[i][m] = pre[i][m];
+    if (ln + nwaves < nlines) {
+#pragma unroll
+      for (int i = 0; i < R::NIN; ++i) load_run<T, V>(A.in[i] + (long long)(ln + nwaves) * n + e0, pre[i]);
+    }
     T wI[2][V], wD[V];
 #pragma unroll
     for (int i = 0; i < R::NIN; ++i) {
-      T own[V];
-      load_run<T, V>(A.in[i] + base, own);
+      T(&own)[V] = cur[i];
       T v[V + 8];
       halo_window<T, V, 4, 4>(own, v, lane);
       if (i == R::I0) {
@@ -251,7 +266,7 @@ static void launch_yz(const ProjArgs<T> &A, const YzShape<T> &Y, hipStream_t st)
 
 template <typename T, int SW>
 static void launch_x(const ProjArgs<T> &A, int V, int nlines, hipStream_t st) {
-  const int waves = nlines < 16384 ? nlines : 16384;
+  const int waves = nlines < 8192 ? nlines : 8192;
   const dim3 grid((waves + 3) / 4), block(256);
   switch (V) {
     case 1: hipLaunchKernelGGL((proj_x_kernel<T, 1, SW>), grid, block, 0, st, A, nlines); break;
