@@ -29,6 +29,12 @@ Parity pins (each checked in ``tests/test_oracle_*.py``):
   known answer p = poisson(div(grad p_exact)) -> p_exact - mean;
 * ``poisson_010``: pinned by the DCT-II closed form (scipy.fft) and by the
   known answer above;
+* ``stretching``: the metric ppy equals yly d(eta)/dy of the mapping yp
+  (4th-order differences on a fine mesh);
+* ``matrice_refinement``: equal, entry by entry, to the even and odd
+  cosine-mode blocks of div(grad .) assembled from the stretched staggered
+  operators; ``inversion5_v1``: equal to dense numpy solves; the stretched
+  ``poisson_010``: inverts div(grad p) up to a constant;
 * TGV diagnostics: pinned by the closed forms at t=0 (KE = 1/8,
   eps = 3 nu / 4, as in examples/TGV-Taylor-Green-vortex/TGV_Re1600.dat row 1)
   and by the analytic enstrophy/eps2 of the initial field;
@@ -69,6 +75,7 @@ class Params:
     dt: float = 0.005
     itimescheme: int = 3
     istret: int = 0
+    beta: float = 0.259
     nu0nu: float = 4.0
     cnu: float = 0.44
 
@@ -331,6 +338,80 @@ class Schemes:
         self.bc = {"x": (p.nclx1, p.nclxn), "y": (p.ncly1, p.nclyn), "z": (p.nclz1, p.nclzn)}
         self.n = {"x": p.nx, "y": p.ny, "z": p.nz}
         self.nm = {"x": p.nxm, "y": p.nym, "z": p.nzm}
+        # mesh stretching in y (xcompact3d init: stretching before schemes)
+        self.stret = stretching(p) if p.istret != 0 else None
+
+
+# ---------------------------------------------------------------------------
+# stretching.f90: the y mapping and its metrics
+# ---------------------------------------------------------------------------
+def stretching(p: Params):
+    """yp, ypi, ppy, pp2y, pp4y, ppyi, pp2yi, pp4yi and alpha
+    (stretching.f90:92-318).  alpha = |xnum/den| is never zero for beta > 0,
+    so only the alpha != 0 branch (:143-226, :247-274) is restated."""
+    ny, yly, beta, ist = p.ny, p.yly, p.beta, p.istret
+    nym = p.nym
+    yinf = -yly / 2.0
+    den = 2.0 * beta * yinf
+    xnum = -yinf - math.sqrt(PI * PI * beta * beta + yinf * yinf)
+    alpha = abs(xnum / den)
+    if alpha == 0.0:
+        raise NotImplementedError("stretching: alpha == 0 branch")
+
+    def mapped(eta):
+        den1 = math.sqrt(alpha * beta + 1.0)
+        xn = den1 / math.sqrt(alpha / PI) / math.sqrt(beta) / math.sqrt(PI)
+        dn = 2.0 * math.sqrt(alpha / PI) * math.sqrt(beta) * PI * math.sqrt(PI)
+        den3 = ((math.sin(PI * eta)) * (math.sin(PI * eta)) / beta / PI) + alpha / PI
+        den4 = 2.0 * alpha * beta - math.cos(2.0 * PI * eta) + 1.0
+        xnum1 = (math.atan(xn * math.tan(PI * eta))) * den4 / den1 / den3 / dn
+        cst = math.sqrt(beta) * PI / (2.0 * math.sqrt(alpha) * math.sqrt(alpha * beta + 1.0))
+        if eta < 0.5:
+            v = xnum1 - cst
+        elif eta == 0.5:
+            v = 0.0
+        else:
+            v = xnum1 + cst
+        if ist == 1:
+            return v - yinf
+        if ist == 2:
+            return v + yly
+        return (v + yly) * 2.0
+
+    yeta = np.zeros(ny)
+    yp = np.zeros(ny)
+    yeta[0] = 0.0 if ist == 1 else -0.5
+    for j in range(1, ny):          # Fortran j = 2..ny
+        if ist == 1:
+            yeta[j] = float(j) * (1.0 / nym)
+        elif ist == 2:
+            yeta[j] = float(j) * (1.0 / nym) - 0.5
+        else:
+            yeta[j] = float(j) * (0.5 / nym) - 0.5
+        yp[j] = mapped(yeta[j])
+    yetai = np.zeros(ny)
+    ypi = np.zeros(ny)
+    for j in range(ny):             # Fortran j = 1..ny
+        if ist == 1:
+            yetai[j] = (float(j + 1) - 0.5) * (1.0 / nym)
+        elif ist == 2:
+            yetai[j] = (float(j + 1) - 0.5) * (1.0 / nym) - 0.5
+        else:
+            yetai[j] = (float(j + 1) - 0.5) * (0.5 / nym) - 0.5
+        ypi[j] = mapped(yetai[j])
+
+    def metrics(eta):
+        s_, c_ = np.sin(PI * eta), np.cos(PI * eta)
+        pp = yly * (alpha / PI + (1.0 / PI / beta) * s_ * s_)
+        p4 = (-2.0 / beta * c_ * s_)
+        if ist == 3:
+            p4 = p4 / 2.0
+        return pp, pp * pp, p4
+
+    ppy, pp2y, pp4y = metrics(yeta)
+    ppyi, pp2yi, pp4yi = metrics(yetai)
+    return dict(alpha=alpha, yeta=yeta, yetai=yetai, yp=yp, ypi=ypi, ppy=ppy, pp2y=pp2y, pp4y=pp4y,
+                ppyi=ppyi, pp2yi=pp2yi, pp4yi=pp4yi)
 
 
 AXIS = {"x": 2, "y": 1, "z": 0}
@@ -656,6 +737,8 @@ def dervp(S: Schemes, u, ax, npaire=0):
         _thomas_cyclic(T, c["cfx6"], c["csx6"], c["cwx6"], c["alcai"])
     else:
         _thomas(T, c["cfx6"], c["csx6"], c["cwx6"])
+    if ax == "y" and S.stret is not None:
+        T = T * S.stret["ppyi"][:S.nm[ax], None]        # derive.f90:4572-4578
     return _unlines(T, (S.nm[ax],) + shape[1:], AXIS[ax])
 
 
@@ -683,6 +766,8 @@ def derpv(S: Schemes, u, ax, npaire=1):
         _thomas_cyclic(T, c["cfx6"], c["csx6"], c["cwx6"], c["alcai"])
     else:
         _thomas(T, c["cfip6"], c["csip6"], c["cwip6"])
+    if ax == "y" and S.stret is not None:
+        T = T * S.stret["ppy"][:, None]                 # derive.f90:4905-4911
     return _unlines(T, (S.n[ax],) + shape[1:], AXIS[ax])
 
 
@@ -827,6 +912,152 @@ def poisson_000_closed_form(S: Schemes, rhs):
     return np.fft.ifftn(ph).real
 
 
+def _refinement_1d(S: Schemes):
+    """The 1-D factors matrice_refinement builds (poisson.f90:1845-1925) for
+    bcx = bcz = 0, bcy = 1: the mid-point interpolation transfer functions
+    transx_rl, transy_rl, transz_rl (at exs*dx, eys*dy, ezs*dz), xk2, zk2, and
+    yky = nym*wp without the 1/yly of the unstretched case (poisson.f90:1614).
+    Real and imaginary parts are equal (cx_one_one), so one real array each."""
+    p = S.p
+    kx2, tx = _wave_1d(S.st["x"], p.dx, p.nx, p.nxm, p.xlx, 0)
+    _, ty = _wave_1d(S.st["y"], p.dy, p.ny, p.nym, p.yly, 1)
+    kz2, tz = _wave_1d(S.st["z"], p.dz, p.nz, p.nzm, p.zlz, 0)
+    c = S.st["y"]
+    yky = np.empty(p.nym)
+    for j in range(p.nym):
+        w = TWOPI * 0.5 * j / p.nym
+        wp = c["aci"] * 2.0 * p.dy * math.sin(w * 0.5) + (c["bci"] * 2.0 * p.dy) * math.sin(1.5 * w)
+        wp = wp / (1.0 + 2.0 * c["alcai"] * math.cos(w))
+        yky[j] = p.nym * wp
+    yky[0] = 0.0
+    nzh = p.nz // 2 + 1
+    return tx, ty, tz[:nzh], kx2, kz2[:nzh], yky
+
+
+def matrice_refinement(S: Schemes):
+    """Pentadiagonal systems of the stretched-y Poisson solve
+    (poisson.f90:1814-2038, istret = 1 or 2, bcz = 0): a for the even cosine
+    modes 0, 2, 4, ... and a2 for the odd modes 1, 3, 5, ..., shape
+    (nz/2+1, nym/2, nx, 5) over (kz, row, kx, diagonal -2..+2).  Entries are
+    real: the reference's real and imaginary parts are equal here."""
+    p = S.p
+    if p.istret not in (1, 2):
+        raise NotImplementedError("matrice_refinement: istret = 3 (inversion5_v2) is not restated")
+    alpha, beta = S.stret["alpha"], p.beta
+    tx, ty, tz, kx2, kz2, yky = _refinement_1d(S)
+    nyh = p.ny // 2                      # the velocity grid's ny/2 = nym/2
+    xa0 = alpha / PI + 0.5 / beta / PI
+    xa1 = (1.0 / 4.0 / beta / PI) if p.istret == 1 else -(1.0 / 4.0 / beta / PI)
+    xa0_2, xa1_2, xa01, xa0p1_2 = xa0 * xa0, xa1 * xa1, xa0 * xa1, (xa0 + xa1) * (xa0 + xa1)
+    TX = tx[None, :]; TZ = tz[:, None]
+    TX2 = TX ** 2; TZ2 = TZ ** 2
+    XK2 = kx2[None, :]; ZK2 = kz2[:, None]
+    ty2 = ty ** 2
+    # cw22(j) = transx * yky(2j-1) * transz, cw2(j) = transx * yky(2j) * transz (1-based j)
+    cw22 = [TX * (yky[2 * j] * TZ) for j in range(nyh)]
+    cw2 = [TX * (yky[2 * j + 1] * TZ) for j in range(nyh)]
+    shape = (tz.size, nyh, tx.size, 5)
+    a = np.zeros(shape)
+    a2 = np.zeros(shape)
+
+    def kxz(t2):
+        return XK2 * t2 * TZ2 + ZK2 * t2 * TX2
+
+    # main diagonal (:1954-2016)
+    for j in range(1, nyh - 1):
+        a[:, j, :, 2] = -(kxz(ty2[2 * j]) + xa0_2 * cw22[j] ** 2 + xa1_2 * cw22[j] * (cw22[j - 1] + cw22[j + 1]))
+        a2[:, j, :, 2] = -(kxz(ty2[2 * j + 1]) + xa0_2 * cw2[j] ** 2 + xa1_2 * cw2[j] * (cw2[j - 1] + cw2[j + 1]))
+    n = nyh - 1
+    a[:, 0, :, 2] = -(kxz(ty2[0]) + xa0_2 * cw22[0] ** 2 + xa1_2 * cw22[0] * cw22[1])
+    a[:, n, :, 2] = -(kxz(ty2[p.ny - 3]) + xa0_2 * cw22[n] ** 2 + xa1_2 * cw22[n] * cw22[n - 1])
+    a2[:, 0, :, 2] = -(kxz(ty2[1]) + (xa0_2 - xa1_2) * cw2[0] ** 2 + xa1_2 * cw2[0] * cw2[1])
+    a2[:, n, :, 2] = -(kxz(ty2[p.ny - 2]) + xa0p1_2 * cw2[n] ** 2 + xa1_2 * cw2[n] * cw2[n - 1])
+    # super-diagonal +1 (:2020-2047)
+    for j in range(1, nyh - 1):
+        a[:, j, :, 3] = xa01 * (cw22[j + 1] * (cw22[j] + cw22[j + 1]))
+        a2[:, j, :, 3] = xa01 * (cw2[j + 1] * (cw2[j] + cw2[j + 1]))
+    a[:, 0, :, 3] = 2.0 * xa01 * (cw22[0] * cw22[1] + cw22[1] * cw22[1])
+    a2[:, 0, :, 3] = (xa0 - xa1) * xa1 * (cw2[0] * cw2[1]) + xa0 * xa1 * (cw2[1] * cw2[1])
+    a2[:, n - 1, :, 3] = xa0 * xa1 * cw2[n - 1] * cw2[n] + (xa0 + xa1) * xa1 * (cw2[n] ** 2)
+    a2[:, n, :, 3] = 0.0
+    # super-diagonal +2 (:2050-2069)
+    for j in range(0, nyh - 2):
+        a[:, j, :, 4] = xa1_2 * (-cw22[j + 1] * cw22[j + 2])
+        a2[:, j, :, 4] = xa1_2 * (-cw2[j + 1] * cw2[j + 2])
+    a[:, 0, :, 4] = 2.0 * a[:, 0, :, 4]
+    a[:, n - 1, :, 4] = 0.0
+    a[:, n, :, 4] = 0.0
+    a2[:, n - 1, :, 4] = 0.0
+    a2[:, n, :, 4] = 0.0
+    # sub-diagonal -1 (:2072-2099)
+    for j in range(1, nyh):
+        a[:, j, :, 1] = xa01 * (cw22[j - 1] * (cw22[j] + cw22[j - 1]))
+        a2[:, j, :, 1] = xa01 * (cw2[j - 1] * (cw2[j] + cw2[j - 1]))
+    a[:, 0, :, 1] = 0.0
+    a2[:, 0, :, 1] = 0.0
+    a2[:, 1, :, 1] = xa0 * xa1 * (cw2[1] * cw2[0]) + (xa0 + xa1) * xa1 * (cw2[0] * cw2[0])
+    a2[:, n, :, 1] = (xa0 + xa1) * xa1 * (cw2[n] * cw2[n - 1]) + xa0 * xa1 * (cw2[n - 1] * cw2[n - 1])
+    # sub-diagonal -2 (:2101-2114)
+    for j in range(2, nyh):
+        a[:, j, :, 0] = xa1_2 * (-cw22[j - 1] * cw22[j - 2])
+        a2[:, j, :, 0] = xa1_2 * (-cw2[j - 1] * cw2[j - 2])
+    a[:, 0:2, :, 0] = 0.0
+    a2[:, 0:2, :, 0] = 0.0
+    # the mean mode (kx = kz = 0): unit first row (:2116-2124)
+    sing = (XK2 == 0.0) & (ZK2 == 0.0)
+    a[:, 0, :, 2] = np.where(sing, 1.0, a[:, 0, :, 2])
+    a[:, 0, :, 3] = np.where(sing, 0.0, a[:, 0, :, 3])
+    a[:, 0, :, 4] = np.where(sing, 0.0, a[:, 0, :, 4])
+    return a, a2
+
+
+EPS_INV5 = 1.0e-16   # tools.f90:1241-1245 (DOUBLE_PREC)
+
+
+def inversion5_v1(aaa_in, eee):
+    """Pentadiagonal solve of tools.f90:1225-1362 on real systems: aaa
+    (nz', m, nx', 5) over (kz, row, kx, diagonal -2..+2), eee (nz', m, nx').
+    Elimination without pivoting, then the last 2x2 block and the back
+    substitution with every division by a pivot of magnitude <= 1e-16 taken
+    as zero.  Where the reference finds an exactly zero pivot during the
+    elimination it keeps the previous line's multiplier; such a line's
+    matrix is zero, its back substitution returns zero either way, and here
+    the multiplier is taken as zero."""
+    aaa = aaa_in.copy()
+    e = eee.copy()
+    m_ = aaa.shape[1]
+    nz = m_ - 1
+    for m in range(m_ - 2):
+        for i in (1, 2):
+            mi = m + i
+            piv = aaa[:, m, :, 2]
+            sr = np.where(piv != 0.0, aaa[:, mi, :, 2 - i] / np.where(piv != 0.0, piv, 1.0), 0.0)
+            e[:, mi, :] = e[:, mi, :] - sr * e[:, m, :]
+            for jc in ((3, 4) if i == 1 else (2, 3)):
+                aaa[:, mi, :, jc - 1] = aaa[:, mi, :, jc - 1] - sr * aaa[:, m, :, jc - 1 + i]
+
+    def guarded(num, den):
+        ok = np.abs(den) > EPS_INV5
+        return np.where(ok, num / np.where(ok, den, 1.0), 0.0), ok
+
+    sr, _ = guarded(aaa[:, nz, :, 1], aaa[:, nz - 1, :, 2])
+    b1 = aaa[:, nz, :, 2] - sr * aaa[:, nz - 1, :, 3]
+    ok = np.abs(b1) > EPS_INV5
+    b1s = np.where(ok, b1, 1.0)
+    t1 = np.where(ok, sr / b1s, 0.0)
+    t3 = np.where(ok, e[:, nz, :] / b1s - t1 * e[:, nz - 1, :], 0.0)
+    e[:, nz, :] = t3
+    rp, _ = guarded(1.0, aaa[:, nz - 1, :, 2])
+    a1 = aaa[:, nz - 1, :, 3] * rp
+    e[:, nz - 1, :] = e[:, nz - 1, :] * rp - a1 * e[:, nz, :]
+    for i in range(m_ - 3, -1, -1):
+        rp, _ = guarded(1.0, aaa[:, i, :, 2])
+        a1 = aaa[:, i, :, 3] * rp
+        b1 = aaa[:, i, :, 4] * rp
+        e[:, i, :] = e[:, i, :] * rp - a1 * e[:, i + 1, :] - b1 * e[:, i + 2, :]
+    return e
+
+
 def poisson_010(S: Schemes, rhs):
     """Poisson with non-periodic y (poisson.f90:665-1013, istret=0 branch),
     restated step by step on the pressure grid (nz, nym, nx)."""
@@ -865,12 +1096,20 @@ def poisson_010(S: Schemes, rhs):
     xx5 = tmp3 * BYv; xx6 = tmp3 * AYv; xx7 = tmp4 * BYv; xx8 = tmp4 * AYv
     c1[:, jj, :] = 0.5 * (xx1 + xx4 + xx5 - xx8)
     c2[:, jj, :] = 0.5 * (-xx2 + xx3 + xx6 + xx7)
-    # divide (poisson.f90:795-819); K real and imag parts are equal
-    K = waves_kxyz(S)[:nzh]
-    small = np.abs(K) < EPS_POISSON
-    Ks = np.where(small, 1.0, K)
-    c1 = np.where(small, 0.0, c1 / (-Ks))
-    c2 = np.where(small, 0.0, c2 / (-Ks))
+    if p.istret == 0:
+        # divide (poisson.f90:795-819); K real and imag parts are equal
+        K = waves_kxyz(S)[:nzh]
+        small = np.abs(K) < EPS_POISSON
+        Ks = np.where(small, 1.0, K)
+        c1 = np.where(small, 0.0, c1 / (-Ks))
+        c2 = np.where(small, 0.0, c2 / (-Ks))
+    else:
+        # stretched y: even and odd cosine modes each solve a pentadiagonal
+        # system, real and imaginary parts separately (poisson.f90:846-879)
+        a, a2 = matrice_refinement(S)
+        for par, A in ((0, a), (1, a2)):
+            c1[:, par::2, :] = inversion5_v1(A, c1[:, par::2, :])
+            c2[:, par::2, :] = inversion5_v1(A, c2[:, par::2, :])
     # zero the (nx/2+1, :, nz/2+1) line (poisson.f90:902-908)
     if nz // 2 < nzh:
         c1[nz // 2, :, nx // 2] = 0.0

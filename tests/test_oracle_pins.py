@@ -231,6 +231,115 @@ def test_poisson_11x_inverts_freeslip_div_grad():
     assert np.abs(pr - (pe - pe.mean())).max() <= 1e-13 * np.abs(pe).max()
 
 
+# ------------------------------------------------------------------ stretched y (istret = 1, 2)
+@pytest.mark.parametrize("istret", [1, 2, 3])
+def test_stretching_metric_is_the_inverse_jacobian(istret):
+    """ppy = yly * d(eta)/dy (twice that for istret = 3, the half channel
+    whose eta spans half the range): checked against 4th-order differences
+    of yp on a fine mesh; yp runs from 0 to yly."""
+    p = O.Params(8, 2049, 8, ncly1=2, nclyn=2, istret=istret, beta=0.7)
+    st = O.stretching(p)
+    yp, deta = st["yp"], st["yeta"][2] - st["yeta"][1]
+    j = np.arange(3, p.ny - 3)
+    dyp = (-yp[j + 2] + 8.0 * yp[j + 1] - 8.0 * yp[j - 1] + yp[j - 2]) / (12.0 * deta)
+    scale = 2.0 if istret == 3 else 1.0
+    # eta = 1/2 (istret = 1) is the one branch switch of the mapping
+    ok = np.abs(st["yeta"][j] - 0.5) > 3 * deta if istret == 1 else np.ones(j.size, bool)
+    r = dyp * st["ppy"][j] / (p.yly * scale)
+    assert np.abs(r[ok] - 1.0).max() < 1e-8
+    assert yp[0] == 0.0 and abs(yp[-1] - p.yly) < 1e-12 * p.yly
+    assert np.all(np.diff(yp) > 0)
+
+
+def _stretched_operator_modes(S):
+    """div(grad .) of the stretched staggered operators on each cosine mode
+    of a y-only field, analysed with scipy's DCT-II: the dense spectral
+    operator for kx = kz = 0."""
+    import scipy.fft as sf
+    p = S.p
+    M = np.zeros((p.nym, p.nym))
+    for m in range(p.nym):
+        e = np.zeros(p.nym)
+        e[m] = 1.0
+        f = np.broadcast_to(sf.idct(e, type=2)[None, :, None], (p.nz, p.nym, p.nx)).copy()
+        M[:, m] = sf.dct(O.divergence(S, *O.gradp(S, f))[0, :, 0], type=2)
+    return M
+
+
+@pytest.mark.parametrize("istret", [1, 2])
+def test_matrice_refinement_is_the_stretched_operator(istret):
+    """The pentadiagonal matrices of matrice_refinement equal, entry by entry,
+    the even and odd cosine-mode blocks of div(grad .) built from the
+    stretched derivative operators (ppy, ppyi metrics), which couple even and
+    odd modes not at all; only the first row of the even block differs, where
+    the reference puts the unit row of the mean mode."""
+    p = O.Params(4, 17, 4, ncly1=1, nclyn=1, istret=istret, beta=0.7)
+    S = O.Schemes(p)
+    M = _stretched_operator_modes(S)
+    assert max(np.abs(M[0::2, 1::2]).max(), np.abs(M[1::2, 0::2]).max()) < 1e-11
+    a, a2 = O.matrice_refinement(S)
+    nyh = p.nym // 2
+
+    def dense(B):
+        D = np.zeros((nyh, nyh))
+        for j in range(nyh):
+            for d in range(5):
+                if 0 <= j + d - 2 < nyh:
+                    D[j, j + d - 2] = B[j, d]
+        return D
+
+    Ae, Ao = dense(a[0, :, 0, :]), dense(a2[0, :, 0, :])
+    Me, Mo = M[0::2, 0::2], M[1::2, 1::2]
+    tol = 1e-11 * np.abs(M).max()
+    assert np.abs(Ae[1:] - Me[1:]).max() < tol
+    assert np.abs(Ao - Mo).max() < tol
+    assert Ae[0, 0] == 1.0 and np.all(Ae[0, 1:] == 0.0)
+
+
+def _no_nyquist(p, f):
+    import scipy.fft as sf
+    F = sf.dct(np.fft.fft(np.fft.fft(f, axis=0), axis=2), type=2, axis=1)
+    F[p.nz // 2, :, :] = 0.0
+    F[:, :, p.nx // 2] = 0.0
+    return np.fft.ifft(np.fft.ifft(sf.idct(F, type=2, axis=1), axis=0), axis=2).real
+
+
+@pytest.mark.parametrize("istret,ncl", [(1, 1), (2, 1), (2, 2)])
+def test_poisson_010_stretched_inverts_div_grad(istret, ncl):
+    """poisson_010 with istret != 0 (matrice_refinement + inversion5_v1)
+    inverts div(grad p) of the stretched staggered operators up to a
+    constant, for p without the Nyquist modes that the mid-point
+    interpolation removes (poisson.f90:846-879)."""
+    p = O.Params(16, 33, 8, ncly1=ncl, nclyn=ncl, istret=istret, beta=0.7)
+    S = O.Schemes(p)
+    pe = _no_nyquist(p, np.random.default_rng(7).standard_normal((p.nz, p.nym, p.nx)))
+    pr = O.poisson(S, O.divergence(S, *O.gradp(S, pe)))
+    assert np.abs((pr - pr.mean()) - (pe - pe.mean())).max() <= 1e-11 * np.abs(pe).max()
+
+
+def test_inversion5_v1_solves_dense_pentadiagonal():
+    """inversion5_v1 equals numpy.linalg.solve on random diagonally dominant
+    pentadiagonal systems."""
+    rng = np.random.default_rng(11)
+    n = 12
+    A = rng.standard_normal((2, n, 3, 5))
+    A[..., 2] += 8.0
+    A[:, 0, :, 0:2] = 0.0
+    A[:, 1, :, 0] = 0.0
+    A[:, n - 1, :, 3:5] = 0.0
+    A[:, n - 2, :, 4] = 0.0
+    e = rng.standard_normal((2, n, 3))
+    x = O.inversion5_v1(A, e)
+    for k in range(2):
+        for i in range(3):
+            D = np.zeros((n, n))
+            for j in range(n):
+                for d in range(5):
+                    if 0 <= j + d - 2 < n:
+                        D[j, j + d - 2] = A[k, j, i, d]
+            assert np.abs(x[k, :, i] - np.linalg.solve(D, e[k, :, i])).max() < 1e-12
+
+
 # ------------------------------------------------------------------ time integration and TGV physics
 def test_ab3_is_exact_for_quadratic_rhs():
     dt = 0.1
