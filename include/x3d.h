@@ -52,8 +52,9 @@ typedef struct x3d_params {
   double re;                       /* xnu = 1/re */
   double dt;
   int itimescheme;                 /* 3 = AB3 (the only scheme of the step driver) */
-  int istret;                      /* 0 only */
+  int istret;                      /* 0 uniform; 1, 2, 3 stretched y (stretching.f90) */
   double nu0nu, cnu;               /* isecondder=5 parameters */
+  double beta;                     /* stretching parameter (istret != 0) */
 } x3d_params;
 
 /* ---- context -------------------------------------------------------------

@@ -26,19 +26,19 @@ class Params(C.Structure):
         ("nclz1", C.c_int), ("nclzn", C.c_int),
         ("ifirstder", C.c_int), ("isecondder", C.c_int), ("ipinter", C.c_int),
         ("re", C.c_double), ("dt", C.c_double), ("itimescheme", C.c_int), ("istret", C.c_int),
-        ("nu0nu", C.c_double), ("cnu", C.c_double),
+        ("nu0nu", C.c_double), ("cnu", C.c_double), ("beta", C.c_double),
     ]
 
 
 def make_params(nx, ny, nz, xlx=None, yly=None, zlz=None, nclx1=0, nclxn=0, ncly1=0, nclyn=0,
                 nclz1=0, nclzn=0, ifirstder=4, isecondder=4, ipinter=3, re=1600.0, dt=0.005,
-                itimescheme=3, istret=0, nu0nu=4.0, cnu=0.44) -> Params:
+                itimescheme=3, istret=0, nu0nu=4.0, cnu=0.44, beta=0.259) -> Params:
     import math
     twopi = 2.0 * math.acos(-1.0)
     return Params(nx, ny, nz, twopi if xlx is None else xlx, twopi if yly is None else yly,
                   twopi if zlz is None else zlz, nclx1, nclxn, ncly1, nclyn, nclz1, nclzn,
                   ifirstder, isecondder, ipinter, float(re), float(dt), itimescheme, istret,
-                  float(nu0nu), float(cnu))
+                  float(nu0nu), float(cnu), float(beta))
 
 
 _lib = None

@@ -96,9 +96,27 @@ int check_dims(const int *a, const int *b, const int *c) {
 }
 
 // derx/dery/derz/derxx/deryy/derzz_<bc1><bcn> (derive.f90:7-3795)
+// The metric a y derivative multiplies by on a stretched mesh (derive.f90:
+// 409-417 ppy for dery, :4572-4578 ppyi for deryvp, :4905-4911 ppy for
+// derypv): the context's device copy.  A caller's host array, when given, must
+// hold the values x3d_init computed (the stretching of the same parameters).
+template <typename T>
+int y_metric(const Context &c, bool pressure, const T *caller, int n, const T **dev) {
+  *dev = nullptr;
+  if (c.p.istret == 0) return X3D_OK;
+  const std::vector<double> &m = pressure ? c.ppyi : c.ppy;
+  if (n > (int)m.size()) return fail(X3D_ERR_ARG, "stretched y: line longer than the metric");
+  if (caller)
+    for (int j = 0; j < n; ++j)
+      if (caller[j] != (T)m[j]) return fail(X3D_ERR_ARG, "stretched y: the metric array differs from x3d_init's");
+  if constexpr (sizeof(T) == 8) *dev = (const T *)(pressure ? c.ppyi64 : c.ppy64);
+  else *dev = (const T *)(pressure ? c.ppyi32 : c.ppy32);
+  return X3D_OK;
+}
+
 template <typename T>
 int der_entry(int dir, int op, int bc1, int bcn, T *t, const T *u, const T *ff, const T *fs, const T *fw,
-              const int *nx, const int *ny, const int *nz, const int *npaire) {
+              const int *nx, const int *ny, const int *nz, const int *npaire, const T *pp = nullptr) {
   Context *c;
   X3D_TRY(require_ctx(&c));
   X3D_TRY(check_dims(nx, ny, nz));
@@ -122,6 +140,11 @@ int der_entry(int dir, int op, int bc1, int bcn, T *t, const T *u, const T *ff, 
   L.bc1 = bc1;
   L.bcn = bcn;
   L.npaire = *npaire;
+  if (dir == 1 && c->p.istret != 0) {
+    if (op != OP_DER1)
+      return fail(X3D_ERR_UNSUPPORTED, "stretched y: deryy (pp2y/pp4y terms) is not provided");
+    X3D_TRY(y_metric<T>(*c, false, pp, n, &L.pp));
+  }
   X3D_TRY(launch_line<T>(L, c->stream));
   X3D_HIP(hipStreamSynchronize(c->stream));
   return X3D_OK;
@@ -131,7 +154,8 @@ int der_entry(int dir, int op, int bc1, int bcn, T *t, const T *u, const T *ff, 
 // module flag of the direction (nclx/ncly/nclz), as in the reference
 template <typename T>
 int stag_entry(int dir, int op, T *t, const T *u, const T *cf1, const T *cs1, const T *cw1, const T *cf2,
-               const T *cs2, const T *cw2, int n1, int n2, int n3, int n_out, int npaire) {
+               const T *cs2, const T *cw2, int n1, int n2, int n3, int n_out, int npaire,
+               const T *pp = nullptr) {
   Context *c;
   X3D_TRY(require_ctx(&c));
   if (!t || !u || !cf1 || !cs1 || !cw1) return fail(X3D_ERR_ARG, "null pointer argument");
@@ -160,12 +184,24 @@ int stag_entry(int dir, int op, T *t, const T *u, const T *cf1, const T *cs1, co
   L.bc1 = D.bc1;
   L.bcn = D.bcn;
   L.npaire = npaire;
+  if (dir == 1 && (op == OP_DERVP || op == OP_DERPV))
+    X3D_TRY(y_metric<T>(*c, op == OP_DERVP, pp, n_out, &L.pp));
   X3D_TRY(launch_line<T>(L, c->stream));
   X3D_HIP(hipStreamSynchronize(c->stream));
   return X3D_OK;
 }
 
 int find_array(const Context &c, const char *name, const std::vector<double> **out) {
+  if (c.p.istret != 0) {
+    // stretching.f90 outputs (the reference's module arrays yp, ypi, ppy, ppyi)
+    const std::string s = name;
+    const std::vector<double> *m = s == "yp" ? &c.yp : s == "ypi" ? &c.ypi : s == "ppy" ? &c.ppy
+                                 : s == "ppyi" ? &c.ppyi : nullptr;
+    if (m) {
+      *out = m;
+      return X3D_OK;
+    }
+  }
   for (int d = 0; d < 3; ++d)
     for (const auto &kv : c.dir[d].arrays)
       if (kv.first == name) {
@@ -188,7 +224,6 @@ int x3d_init(const x3d_params *p) {
   std::lock_guard<std::mutex> lk(g_ctx_mu);
   if (!p) return fail(X3D_ERR_ARG, "null params");
   if (g_ctx) return fail(X3D_ERR_STATE, "x3d_init called twice (call x3d_finalize first)");
-  if (p->istret != 0) return fail(X3D_ERR_UNSUPPORTED, "istret != 0 (stretched y mesh) is not provided");
   auto c = std::make_unique<Context>();
   c->p = *p;
   if (c->p.ipinter == 0) c->p.ipinter = 3;   // parameters.f90:708 default
@@ -197,11 +232,20 @@ int x3d_init(const x3d_params *p) {
   X3D_HIP(hipGetDevice(&c->device));
   X3D_TRY(build_schemes(*c));
   X3D_TRY(upload_sets(*c));
+  {
+    int r = build_stretching(*c);
+    if (r != X3D_OK) {
+      free_stretching(*c);
+      free_sets(*c);
+      return r;
+    }
+  }
   // a grid without a Poisson solver (poisson_100, sizes the FFT passes do
   // not cover) still gets the derivative entries
   {
     int r = poisson_setup(*c);
     if (r != X3D_OK && r != X3D_ERR_UNSUPPORTED) {
+      free_stretching(*c);
       free_sets(*c);
       return r;
     }
@@ -215,6 +259,7 @@ int x3d_finalize(void) {
   if (!g_ctx) return X3D_OK;
   (void)hipDeviceSynchronize();
   poisson_free(*g_ctx);
+  free_stretching(*g_ctx);
   free_sets(*g_ctx);
   for (TriSet &f : g_foreign) {
     if (f.d64) (void)hipFree(f.d64);
@@ -315,8 +360,8 @@ int x3d_get_scalar(const char *name, double *out) {
 #define X3D_DEF_DERY(fname, T, b1, bn)                                                             \
   int fname(T *t, const T *u, T *r, T *s, const T *ff, const T *fs, const T *fw, const T *pp,    \
             const int *nx, const int *ny, const int *nz, const int *npaire, const T *lind) {     \
-    (void)r; (void)s; (void)lind; (void)pp; /* istret=0: dery does not apply ppy */              \
-    return der_entry<T>(1, OP_DER1, b1, bn, t, u, ff, fs, fw, nx, ny, nz, npaire);               \
+    (void)r; (void)s; (void)lind; /* pp: ppy, applied when istret != 0 */                        \
+    return der_entry<T>(1, OP_DER1, b1, bn, t, u, ff, fs, fw, nx, ny, nz, npaire, pp);           \
   }
 #define X3D_DEF_FAMILY(T, SFX)                                                  \
   X3D_DEF_DERX(x3d_derx_00##SFX, T, 0, OP_DER1, 0, 0)                           \
@@ -391,9 +436,10 @@ X3D_DEF_FAMILY(float, _f32)
   }                                                                                                              \
   int x3d_deryvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs, const T *cw, const T *ppyi,        \
                       const int *nx, const int *ny, const int *nym, const int *nz, const int *npaire) {          \
-    (void)r; (void)s; (void)ppyi;                                                                                \
+    (void)r; (void)s;                                                                                            \
     X3D_TRY(check_dims(nx, ny, nz)); X3D_TRY(check_dims(nym, nym, npaire ? nym : nullptr));                      \
-    return stag_entry<T>(1, OP_DERVP, t, u, cf, cs, cw, nullptr, nullptr, nullptr, *nx, *ny, *nz, *nym, *npaire); \
+    return stag_entry<T>(1, OP_DERVP, t, u, cf, cs, cw, nullptr, nullptr, nullptr, *nx, *ny, *nz, *nym, *npaire, \
+                         ppyi);                                                                                  \
   }                                                                                                              \
   int x3d_interypv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1, const T *cw1, const T *cf2,    \
                         const T *cs2, const T *cw2, const int *nx, const int *nym, const int *ny, const int *nz, \
@@ -405,9 +451,9 @@ X3D_DEF_FAMILY(float, _f32)
   int x3d_derypv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1, const T *cw1, const T *cf2,      \
                       const T *cs2, const T *cw2, const T *ppy, const int *nx, const int *nym, const int *ny,    \
                       const int *nz, const int *npaire) {                                                        \
-    (void)r; (void)s; (void)ppy;                                                                                 \
+    (void)r; (void)s;                                                                                            \
     X3D_TRY(check_dims(nx, nym, nz)); X3D_TRY(check_dims(ny, ny, npaire ? ny : nullptr));                        \
-    return stag_entry<T>(1, OP_DERPV, t, u, cf1, cs1, cw1, cf2, cs2, cw2, *nx, *nym, *nz, *ny, *npaire);         \
+    return stag_entry<T>(1, OP_DERPV, t, u, cf1, cs1, cw1, cf2, cs2, cw2, *nx, *nym, *nz, *ny, *npaire, ppy);    \
   }                                                                                                              \
   int x3d_derzvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs, const T *cw, const int *nx,        \
                       const int *ny, const int *nz, const int *nzm, const int *npaire) {                         \

@@ -70,6 +70,11 @@ struct PoissonPlan {
   PoissonTables<double> t64;
   PoissonTables<float> t32;
   void *cw64 = nullptr, *cw32 = nullptr;   // spectral work arrays
+  // stretched y (poisson_010 with istret = 1, 2): factors of the pentadiagonal
+  // systems, five FP64 arrays [kz][parity][row][kx] (see build_penta_factors)
+  int ystret = 0;
+  double *pf = nullptr;
+  size_t pf_stride = 0;
 };
 
 static int ilog2(int n) {
@@ -402,7 +407,12 @@ __global__ void __launch_bounds__(FFT_THREADS) c2r_x_kernel(const cplx<T> *__res
 }
 
 // ---------------------------------------------------------------- passes 2-4: strided lines
-enum { FFT_Y_FWD = 0, FFT_Y_INV = 1, FFT_Z_SOLVE = 2 };
+// Z_FWD / Z_INV: the z transform alone, around the stretched-y solve
+enum { FFT_Y_FWD = 0, FFT_Y_INV = 1, FFT_Z_SOLVE = 2, FFT_Z_FWD = 3, FFT_Z_INV = 4 };
+template <int MODE>
+struct ZPass {
+  static constexpr bool value = MODE == FFT_Z_SOLVE || MODE == FFT_Z_FWD || MODE == FFT_Z_INV;
+};
 
 
 
@@ -439,12 +449,13 @@ __global__ void __launch_bounds__((SShape<T, N>::threads(EXTRA)))
     a = nmain + gg / nplanes;
     b = gg % nplanes;
   }
-  const size_t se = MODE == FFT_Z_SOLVE ? (size_t)nxh * ny : (size_t)nxh;
-  const size_t base = MODE == FFT_Z_SOLVE ? (size_t)b * nxh + a : (size_t)b * nxh * ny + a;
-  stage_table(tw, MODE == FFT_Z_SOLVE ? tb.twz : tb.twy, N);
+  constexpr bool ZL = ZPass<MODE>::value;
+  const size_t se = ZL ? (size_t)nxh * ny : (size_t)nxh;
+  const size_t base = ZL ? (size_t)b * nxh + a : (size_t)b * nxh * ny + a;
+  stage_table(tw, ZL ? tb.twz : tb.twy, N);
   // the cosine transform of this pass: y in the y passes, z in the z pass
-  const bool dct_here = MODE == FFT_Z_SOLVE ? (COS && zdct != 0) : ydct != 0;
-  if (dct_here) stage_table(dct, MODE == FFT_Z_SOLVE ? tb.dctz : tb.dcty, N);
+  const bool dct_here = ZL ? (MODE == FFT_Z_SOLVE && COS && zdct != 0) : ydct != 0;
+  if (dct_here) stage_table(dct, ZL ? tb.dctz : tb.dcty, N);
   const bool reorder_in = (MODE == FFT_Y_FWD || MODE == FFT_Z_SOLVE) && dct_here;
   cplx<T> x[E];
 #pragma unroll
@@ -473,7 +484,7 @@ __global__ void __launch_bounds__((SShape<T, N>::threads(EXTRA)))
   }
   __syncthreads();   // tables staged
   const LdRows<T> ld{buf, TLB, l};
-  fft_stages<T, N, MODE == FFT_Y_INV, 1>(x, t, ld, tw);
+  fft_stages<T, N, MODE == FFT_Y_INV || MODE == FFT_Z_INV, 1>(x, t, ld, tw);
 
   if ((MODE == FFT_Y_FWD || MODE == FFT_Z_SOLVE) && dct_here) {
     // DCT-II output side: C_k = (e^{-i pi k/2N} V_k + e^{+i pi k/2N} V_{N-k}) / 2,
@@ -744,6 +755,179 @@ static int build_tables(const PoissonPlan &P, const std::vector<double> *tabs, v
   return X3D_OK;
 }
 
+// ---------------------------------------------------------------- stretched y
+// poisson_010 with istret = 1, 2 (poisson.f90:846-879): after the x, y
+// (cosine) and z transforms, the even cosine modes 0, 2, 4, ... and the odd
+// modes 1, 3, 5, ... of each (kx, kz) line solve the pentadiagonal systems of
+// matrice_refinement (poisson.f90:1814-2038) with inversion5_v1
+// (tools.f90:1225-1362).  Their matrices are real (the reference's real and
+// imaginary parts are equal for periodic x and z) and do not depend on the
+// right-hand side, so the elimination runs once here, on the host in FP64, in
+// the reference's order, and the kernel applies the stored factors:
+//   F1[m], F2[m]  multipliers of rows m+1, m+2 by row m (m < nh-2)
+//   RP[m], A1[m], B1[m]  back substitution e_m = e_m RP - A1 e_{m+1} - B1 e_{m+2}
+//   last row n = nh-1: RP[n] = b1 (0 when |b1| <= 1e-16), A1[n] = sr/b1,
+//   e_n = e_n/b1 - A1 e_{n-1}
+// A pivot of magnitude <= 1e-16 gives a zero factor, as the reference's
+// guarded divisions; an exactly zero pivot during the elimination (a line
+// whose matrix is zero) gives a zero multiplier.
+static void build_penta_factors(const Context &c, const PoissonPlan &P, const std::vector<double> &kx2,
+                                const std::vector<double> &tx, const std::vector<double> &ty,
+                                const std::vector<double> &kz2, const std::vector<double> &tz,
+                                std::vector<double> &out) {
+  const DirCoef &Y = c.dir[1];
+  const double pi = std::acos(-1.0);
+  const int nym = P.ny, nh = nym / 2, nxh = P.nxh, nz = P.nz;
+  const double eps = 1.0e-16;
+  // yky = nym wp without 1/yly on a stretched mesh (poisson.f90:1614)
+  std::vector<double> yky(nym, 0.0);
+  for (int j = 1; j < nym; ++j) {
+    const double w = 2.0 * pi * 0.5 * j / nym;
+    double wp = Y.s.aci * 2.0 * Y.d * std::sin(w * 0.5) + (Y.s.bci * 2.0 * Y.d) * std::sin(1.5 * w);
+    wp = wp / (1.0 + 2.0 * Y.s.alcai * std::cos(w));
+    yky[j] = nym * wp;
+  }
+  const double alpha = c.alpha, beta = c.p.beta;
+  const double xa0 = alpha / pi + 0.5 / beta / pi;
+  const double xa1 = c.p.istret == 1 ? 1.0 / 4.0 / beta / pi : -(1.0 / 4.0 / beta / pi);
+  const double xa0_2 = xa0 * xa0, xa1_2 = xa1 * xa1, xa01 = xa0 * xa1, xa0p1_2 = (xa0 + xa1) * (xa0 + xa1);
+  const size_t stride = (size_t)nz * 2 * nh * nxh;
+  out.assign(5 * stride, 0.0);
+  std::vector<double> cw(nh), a(5 * nh);
+  for (int kz = 0; kz < nz; ++kz)
+    for (int par = 0; par < 2; ++par)
+      for (int kx = 0; kx < nxh; ++kx) {
+        const double TX = tx[kx], TZ = tz[kz], TX2 = TX * TX, TZ2 = TZ * TZ, XK2 = kx2[kx], ZK2 = kz2[kz];
+        for (int j = 0; j < nh; ++j) cw[j] = TX * (yky[2 * j + par] * TZ);
+        auto A = [&](int j, int d) -> double & { return a[5 * j + d]; };
+        std::fill(a.begin(), a.end(), 0.0);
+        auto kxz = [&](int mode) { const double t2 = ty[mode] * ty[mode]; return XK2 * t2 * TZ2 + ZK2 * t2 * TX2; };
+        const int n = nh - 1;
+        // main diagonal
+        for (int j = 1; j < n; ++j)
+          A(j, 2) = -(kxz(2 * j + par) + xa0_2 * (cw[j] * cw[j]) + xa1_2 * cw[j] * (cw[j - 1] + cw[j + 1]));
+        if (par == 0) {
+          A(0, 2) = -(kxz(0) + xa0_2 * (cw[0] * cw[0]) + xa1_2 * cw[0] * cw[1]);
+          A(n, 2) = -(kxz(2 * n) + xa0_2 * (cw[n] * cw[n]) + xa1_2 * cw[n] * cw[n - 1]);
+        } else {
+          A(0, 2) = -(kxz(1) + (xa0_2 - xa1_2) * (cw[0] * cw[0]) + xa1_2 * cw[0] * cw[1]);
+          A(n, 2) = -(kxz(2 * n + 1) + xa0p1_2 * (cw[n] * cw[n]) + xa1_2 * cw[n] * cw[n - 1]);
+        }
+        // +1
+        for (int j = 1; j < n; ++j) A(j, 3) = xa01 * (cw[j + 1] * (cw[j] + cw[j + 1]));
+        if (par == 0) {
+          A(0, 3) = 2.0 * xa01 * (cw[0] * cw[1] + cw[1] * cw[1]);
+        } else {
+          A(0, 3) = (xa0 - xa1) * xa1 * (cw[0] * cw[1]) + xa0 * xa1 * (cw[1] * cw[1]);
+          A(n - 1, 3) = xa0 * xa1 * cw[n - 1] * cw[n] + (xa0 + xa1) * xa1 * (cw[n] * cw[n]);
+          A(n, 3) = 0.0;
+        }
+        // +2
+        for (int j = 0; j < nh - 2; ++j) A(j, 4) = xa1_2 * (-cw[j + 1] * cw[j + 2]);
+        if (par == 0) A(0, 4) = 2.0 * A(0, 4);
+        A(n - 1, 4) = 0.0;
+        A(n, 4) = 0.0;
+        // -1
+        for (int j = 1; j < nh; ++j) A(j, 1) = xa01 * (cw[j - 1] * (cw[j] + cw[j - 1]));
+        A(0, 1) = 0.0;
+        if (par == 1) {
+          A(1, 1) = xa0 * xa1 * (cw[1] * cw[0]) + (xa0 + xa1) * xa1 * (cw[0] * cw[0]);
+          A(n, 1) = (xa0 + xa1) * xa1 * (cw[n] * cw[n - 1]) + xa0 * xa1 * (cw[n - 1] * cw[n - 1]);
+        }
+        // -2
+        for (int j = 2; j < nh; ++j) A(j, 0) = xa1_2 * (-cw[j - 1] * cw[j - 2]);
+        // the mean mode: unit first row of the even system
+        if (par == 0 && XK2 == 0.0 && ZK2 == 0.0) {
+          A(0, 2) = 1.0;
+          A(0, 3) = 0.0;
+          A(0, 4) = 0.0;
+        }
+        // inversion5_v1's elimination, recording the factors
+        const size_t line = ((size_t)(kz * 2 + par) * nh) * nxh + kx;
+        auto F = [&](int q, int m) -> double & { return out[q * stride + line + (size_t)m * nxh]; };
+        for (int m = 0; m + 2 < nh; ++m) {
+          const double piv = A(m, 2);
+          const double s1 = piv != 0.0 ? A(m + 1, 1) / piv : 0.0;
+          A(m + 1, 2) = A(m + 1, 2) - s1 * A(m, 3);
+          A(m + 1, 3) = A(m + 1, 3) - s1 * A(m, 4);
+          const double s2 = piv != 0.0 ? A(m + 2, 0) / piv : 0.0;
+          A(m + 2, 1) = A(m + 2, 1) - s2 * A(m, 3);
+          A(m + 2, 2) = A(m + 2, 2) - s2 * A(m, 4);
+          F(0, m) = s1;
+          F(1, m) = s2;
+        }
+        {
+          const double sr = std::fabs(A(n - 1, 2)) > eps ? A(n, 1) / A(n - 1, 2) : 0.0;
+          const double b1 = A(n, 2) - sr * A(n - 1, 3);
+          const bool ok = std::fabs(b1) > eps;
+          F(2, n) = ok ? b1 : 0.0;
+          F(3, n) = ok ? sr / b1 : 0.0;
+          const double rp = std::fabs(A(n - 1, 2)) > eps ? 1.0 / A(n - 1, 2) : 0.0;
+          F(2, n - 1) = rp;
+          F(3, n - 1) = A(n - 1, 3) * rp;
+        }
+        for (int i = nh - 3; i >= 0; --i) {
+          const double rp = std::fabs(A(i, 2)) > eps ? 1.0 / A(i, 2) : 0.0;
+          F(2, i) = rp;
+          F(3, i) = A(i, 3) * rp;
+          F(4, i) = A(i, 4) * rp;
+        }
+      }
+}
+
+// One thread per (kx, kz, parity): forward elimination with the stored
+// multipliers (the scaled right-hand side read once, the eliminated values
+// written back), then the last 2x2 block and the back substitution.  The
+// (nx/2+1, :, nz/2+1) line is zeroed (poisson.f90:902-908).
+template <typename T>
+__global__ void __launch_bounds__(64) penta_y_kernel(cplx<T> *__restrict__ cw, const double *__restrict__ pf,
+                                                     size_t stride, int nxh, int ny, int nz, double scale) {
+  const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int kz = blockIdx.y >> 1, par = blockIdx.y & 1;
+  if (kx >= nxh) return;
+  const int nh = ny >> 1, n = nh - 1;
+  const size_t line = ((size_t)blockIdx.y * nh) * nxh + kx;
+  const double *F1 = pf + line, *F2 = pf + stride + line, *RP = pf + 2 * stride + line;
+  const double *A1 = pf + 3 * stride + line, *B1 = pf + 4 * stride + line;
+  auto at = [&](int m) -> cplx<T> & { return cw[((size_t)kz * ny + (2 * m + par)) * nxh + kx]; };
+  if (kx == nxh - 1 && kz == (nz >> 1)) {
+    for (int m = 0; m < nh; ++m) at(m) = cplx<T>{T(0), T(0)};
+    return;
+  }
+  auto ld = [&](int m) {
+    const cplx<T> v = at(m);
+    return cplx<double>{(double)v.x * scale, (double)v.y * scale};
+  };
+  auto st = [&](int m, cplx<double> v) { at(m) = cplx<T>{(T)v.x, (T)v.y}; };
+  // forward: window (e0, e1, e2) = rows m, m+1, m+2
+  cplx<double> e0 = ld(0), e1 = ld(1), e2;
+  for (int m = 0; m + 2 < nh; ++m) {
+    e2 = ld(m + 2);
+    const double s1 = F1[(size_t)m * nxh], s2 = F2[(size_t)m * nxh];
+    e1 = cplx<double>{e1.x - s1 * e0.x, e1.y - s1 * e0.y};
+    e2 = cplx<double>{e2.x - s2 * e0.x, e2.y - s2 * e0.y};
+    st(m, e0);
+    e0 = e1;
+    e1 = e2;
+  }
+  // e0, e1 = rows n-1, n
+  const double b1 = RP[(size_t)n * nxh], t1 = A1[(size_t)n * nxh];
+  cplx<double> xn = b1 != 0.0 ? cplx<double>{e1.x / b1 - t1 * e0.x, e1.y / b1 - t1 * e0.y} : cplx<double>{0.0, 0.0};
+  const double rp = RP[(size_t)(n - 1) * nxh], a1 = A1[(size_t)(n - 1) * nxh];
+  cplx<double> xm = {e0.x * rp - a1 * xn.x, e0.y * rp - a1 * xn.y};
+  st(n, xn);
+  st(n - 1, xm);
+  // back substitution: (xm, xn) = rows i+1, i+2
+  for (int i = nh - 3; i >= 0; --i) {
+    const cplx<T> v = at(i);
+    const double r = RP[(size_t)i * nxh], a = A1[(size_t)i * nxh], bb = B1[(size_t)i * nxh];
+    const cplx<double> x = {((double)v.x * r - a * xm.x) - bb * xn.x, ((double)v.y * r - a * xm.y) - bb * xn.y};
+    st(i, x);
+    xn = xm;
+    xm = x;
+  }
+}
+
 int poisson_setup(Context &c) {
   const DirCoef &X = c.dir[0], &Y = c.dir[1], &Z = c.dir[2];
   // dispatch of decomp_2d_poisson_init (poisson.f90:98-108): 000, 010 and
@@ -780,9 +964,29 @@ int poisson_setup(Context &c) {
     tabs[0].resize(P->nxh);
     tabs[1].resize(P->nxh);
   }
+  if (c.p.istret != 0) {
+    // stretched y: poisson_010's pentadiagonal branch (istret = 3 uses
+    // inversion5_v2 on the whole line, not provided)
+    if (P->xdct || P->zdct || !P->ydct || (c.p.istret != 1 && c.p.istret != 2)) {
+      delete P;
+      return fail(X3D_ERR_UNSUPPORTED, "Poisson on a stretched y mesh: poisson_010 with istret = 1 or 2 only");
+    }
+    std::vector<double> f;
+    build_penta_factors(c, *P, tabs[0], tabs[1], tabs[3], tabs[4], tabs[5], f);
+    P->ystret = 1;
+    P->pf_stride = f.size() / 5;
+    hipError_t e = hipMalloc(&P->pf, f.size() * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(P->pf, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (P->pf) (void)hipFree(P->pf);
+      delete P;
+      return fail(X3D_ERR_HIP, std::string("Poisson factors: ") + hipGetErrorString(e));
+    }
+  }
   int r = build_tables<double>(*P, tabs, &P->tab64, &P->t64);
   if (r == X3D_OK) r = build_tables<float>(*P, tabs, &P->tab32, &P->t32);
   if (r != X3D_OK) {
+    if (P->pf) (void)hipFree(P->pf);
     delete P;
     return r;
   }
@@ -797,6 +1001,7 @@ void poisson_free(Context &c) {
   if (P->tab32) (void)hipFree(P->tab32);
   if (P->cw64) (void)hipFree(P->cw64);
   if (P->cw32) (void)hipFree(P->cw32);
+  if (P->pf) (void)hipFree(P->pf);
   delete P;
   c.poisson = nullptr;
 }
@@ -815,7 +1020,7 @@ template <typename T, int N, int MODE, bool COS, bool EXTRA>
 static void launch_s1(const PoissonPlan &P, cplx<T> *cw, const PoissonTables<T> &tb, int nmain, int ncols,
                       hipStream_t st) {
   using S = SShape<T, N>;
-  const dim3 grid(ncols, MODE == FFT_Z_SOLVE ? P.ny : P.nz), block(S::threads(EXTRA));
+  const dim3 grid(ncols, ZPass<MODE>::value ? P.ny : P.nz), block(S::threads(EXTRA));
   // poisson_010 zeroes the (nx/2+1, :, nz/2+1) line (poisson.f90:902-908); poisson_11x does not
   const int zero_line = MODE == FFT_Z_SOLVE && P.ydct && !P.xdct;
   hipLaunchKernelGGL((strided_kernel<T, N, MODE, COS, EXTRA>), grid, block, S::lds(EXTRA), st, cw, nmain, P.nxh,
@@ -884,7 +1089,18 @@ int poisson_solve(Context &c, T *rhs, hipStream_t st) {
   // sizes were checked by poisson_setup: every dispatch below finds its case
   bool ok = dispatch_x<T>(true, M, rhs, cw, nlines, P->xdct, tb, st);
   ok = ok && dispatch_s<T, FFT_Y_FWD>(P->ny, *P, cw, tb, st);
-  ok = ok && dispatch_s<T, FFT_Z_SOLVE>(P->nz, *P, cw, tb, st);
+  if (P->ystret) {
+    // stretched y: z transform, pentadiagonal solves along y, inverse z
+    ok = ok && dispatch_s<T, FFT_Z_FWD>(P->nz, *P, cw, tb, st);
+    if (ok) {
+      const dim3 grid((P->nxh + 63) / 64, 2 * P->nz), block(64);
+      hipLaunchKernelGGL((penta_y_kernel<T>), grid, block, 0, st, cw, P->pf, P->pf_stride, P->nxh, P->ny, P->nz,
+                         P->scale);
+    }
+    ok = ok && dispatch_s<T, FFT_Z_INV>(P->nz, *P, cw, tb, st);
+  } else {
+    ok = ok && dispatch_s<T, FFT_Z_SOLVE>(P->nz, *P, cw, tb, st);
+  }
   ok = ok && dispatch_s<T, FFT_Y_INV>(P->ny, *P, cw, tb, st);
   ok = ok && dispatch_x<T>(false, M, rhs, cw, nlines, P->xdct, tb, st);
   if (!ok) return fail(X3D_ERR_UNSUPPORTED, "Poisson: no FFT kernel for this size");

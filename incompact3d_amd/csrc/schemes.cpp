@@ -510,4 +510,82 @@ int make_line_call(const Context &c, int dir, int op, int npaire, LineCall<T> *o
 template int make_line_call<double>(const Context &, int, int, int, LineCall<double> *);
 template int make_line_call<float>(const Context &, int, int, int, LineCall<float> *);
 
+// ---------------------------------------------------------------- stretched y
+// stretching.f90:92-318 (alpha != 0: alpha = |xnum/den| > 0 for beta > 0).
+// yeta/yetai are the computational coordinates of the velocity and pressure
+// points, yp/ypi the physical ones, ppy/ppyi = yly (alpha/pi + sin^2(pi
+// eta)/(pi beta)) the metric that the y derivatives multiply by.
+int build_stretching(Context &c) {
+  const x3d_params &p = c.p;
+  if (p.istret == 0) return X3D_OK;
+  if (p.istret < 0 || p.istret > 3) return fail(X3D_ERR_ARG, "istret must be 0, 1, 2 or 3");
+  if (!(p.beta > 0.0)) return fail(X3D_ERR_ARG, "stretching needs beta > 0");
+  if (p.ncly1 == 0 && p.nclyn == 0)
+    return fail(X3D_ERR_UNSUPPORTED, "stretched y needs non-periodic y boundaries");
+  const double pi = std::acos(-1.0), yly = p.yly, beta = p.beta;
+  const int ny = p.ny, nym = c.dir[1].nm, ist = p.istret;
+  const double yinf = -yly / 2.0;
+  const double den = 2.0 * beta * yinf;
+  const double xnum = -yinf - std::sqrt(pi * pi * beta * beta + yinf * yinf);
+  const double alpha = std::fabs(xnum / den);
+  c.alpha = alpha;
+  auto mapped = [&](double eta) {
+    const double den1 = std::sqrt(alpha * beta + 1.0);
+    const double xn = den1 / std::sqrt(alpha / pi) / std::sqrt(beta) / std::sqrt(pi);
+    const double dn = 2.0 * std::sqrt(alpha / pi) * std::sqrt(beta) * pi * std::sqrt(pi);
+    const double den3 = ((std::sin(pi * eta)) * (std::sin(pi * eta)) / beta / pi) + alpha / pi;
+    const double den4 = 2.0 * alpha * beta - std::cos(2.0 * pi * eta) + 1.0;
+    const double xnum1 = (std::atan(xn * std::tan(pi * eta))) * den4 / den1 / den3 / dn;
+    const double cst = std::sqrt(beta) * pi / (2.0 * std::sqrt(alpha) * std::sqrt(alpha * beta + 1.0));
+    const double v = eta < 0.5 ? xnum1 - cst : (eta == 0.5 ? 0.0 : xnum1 + cst);
+    if (ist == 1) return v - yinf;
+    if (ist == 2) return v + yly;
+    return (v + yly) * 2.0;
+  };
+  std::vector<double> yeta(ny), yetai(ny);
+  c.yp.assign(ny, 0.0);
+  c.ypi.assign(ny, 0.0);
+  yeta[0] = ist == 1 ? 0.0 : -0.5;
+  for (int j = 1; j < ny; ++j) {
+    if (ist == 1) yeta[j] = (double)j * (1.0 / nym);
+    else if (ist == 2) yeta[j] = (double)j * (1.0 / nym) - 0.5;
+    else yeta[j] = (double)j * (0.5 / nym) - 0.5;
+    c.yp[j] = mapped(yeta[j]);
+  }
+  for (int j = 0; j < ny; ++j) {
+    if (ist == 1) yetai[j] = ((double)(j + 1) - 0.5) * (1.0 / nym);
+    else if (ist == 2) yetai[j] = ((double)(j + 1) - 0.5) * (1.0 / nym) - 0.5;
+    else yetai[j] = ((double)(j + 1) - 0.5) * (0.5 / nym) - 0.5;
+    c.ypi[j] = mapped(yetai[j]);
+  }
+  c.ppy.assign(ny, 0.0);
+  c.ppyi.assign(ny, 0.0);
+  for (int j = 0; j < ny; ++j) {
+    const double s1 = std::sin(pi * yeta[j]), s2 = std::sin(pi * yetai[j]);
+    c.ppy[j] = yly * (alpha / pi + (1.0 / pi / beta) * s1 * s1);
+    c.ppyi[j] = yly * (alpha / pi + (1.0 / pi / beta) * s2 * s2);
+  }
+  std::vector<float> f(ny);
+  X3D_HIP(hipMalloc(&c.ppy64, ny * sizeof(double)));
+  X3D_HIP(hipMalloc(&c.ppyi64, ny * sizeof(double)));
+  X3D_HIP(hipMalloc(&c.ppy32, ny * sizeof(float)));
+  X3D_HIP(hipMalloc(&c.ppyi32, ny * sizeof(float)));
+  X3D_HIP(hipMemcpy(c.ppy64, c.ppy.data(), ny * sizeof(double), hipMemcpyHostToDevice));
+  X3D_HIP(hipMemcpy(c.ppyi64, c.ppyi.data(), ny * sizeof(double), hipMemcpyHostToDevice));
+  for (int j = 0; j < ny; ++j) f[j] = (float)c.ppy[j];
+  X3D_HIP(hipMemcpy(c.ppy32, f.data(), ny * sizeof(float), hipMemcpyHostToDevice));
+  for (int j = 0; j < ny; ++j) f[j] = (float)c.ppyi[j];
+  X3D_HIP(hipMemcpy(c.ppyi32, f.data(), ny * sizeof(float), hipMemcpyHostToDevice));
+  return X3D_OK;
+}
+
+void free_stretching(Context &c) {
+  if (c.ppy64) (void)hipFree(c.ppy64);
+  if (c.ppyi64) (void)hipFree(c.ppyi64);
+  if (c.ppy32) (void)hipFree(c.ppy32);
+  if (c.ppyi32) (void)hipFree(c.ppyi32);
+  c.ppy64 = c.ppyi64 = nullptr;
+  c.ppy32 = c.ppyi32 = nullptr;
+}
+
 }  // namespace x3d

@@ -522,6 +522,8 @@ int x3d_solver_create(int precision, x3d_solver **out) {
       return fail(X3D_ERR_UNSUPPORTED, "the device time step covers periodic and free-slip (ncl=1) boundaries");
   }
   if (!c->poisson) return fail(X3D_ERR_UNSUPPORTED, "no Poisson solver for this grid");
+  if (c->p.istret != 0)
+    return fail(X3D_ERR_UNSUPPORTED, "the device time step has no stretched-y momentum terms (istret != 0)");
   if (c->p.itimescheme != 3 && c->p.itimescheme != 5)
     return fail(X3D_ERR_UNSUPPORTED, "the device time step implements AB3 (itimescheme=3) and RK3 (itimescheme=5)");
   auto *S = new x3d_solver();

@@ -104,6 +104,12 @@ struct Context {
   std::vector<TriSet> sets;
   hipStream_t stream = nullptr;
   int device = 0;
+  // stretched y mesh (istret != 0): the metrics of stretching.f90 on the
+  // velocity (ppy) and pressure (ppyi) grids, host FP64 and device copies
+  double alpha = 0;
+  std::vector<double> yp, ypi, ppy, ppyi;
+  double *ppy64 = nullptr, *ppyi64 = nullptr;
+  float *ppy32 = nullptr, *ppyi32 = nullptr;
   // Poisson state (fft.hip)
   void *poisson = nullptr;
   // scratch for host-memory entries
@@ -116,6 +122,8 @@ int require_ctx(Context **c);
 
 // schemes.cpp
 int build_schemes(Context &c);
+int build_stretching(Context &c);   // istret != 0: metrics on host and device
+void free_stretching(Context &c);
 int upload_sets(Context &c);
 void free_sets(Context &c);
 const TriSet *match_set(const Context &c, int dir, int n, const double *ff, const double *fs,

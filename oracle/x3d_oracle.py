@@ -706,6 +706,8 @@ def der1(S: Schemes, u, ax, npaire):
         _thomas_cyclic(T, ff, fs, fw, c["alfai"])
     else:
         _thomas(T, ff, fs, fw)
+    if ax == "y" and S.stret is not None:
+        T = T * S.stret["ppy"][:, None]                 # derive.f90:409-417
     return _unlines(T, shape, AXIS[ax])
 
 
