@@ -224,6 +224,33 @@ def microbench(X, n, walls, stream, reps=10):
     return {"grid": [n, ny, n], "y": "Dirichlet walls (ncl=2)" if walls else "periodic", "ops": out}
 
 
+def stretched_poisson_bench(X, stream, reps=10):
+    """Pressure Poisson solve on the channel grid 256 x 129 x 128 (walls in y,
+    ncl=2), uniform (poisson_010, istret=0) and stretched (istret=2,
+    beta=0.259: matrice_refinement's pentadiagonal systems), FP64 and FP32,
+    HIP-event averages over `reps` solves."""
+    import torch
+    out = {"grid": [256, 129, 128], "y": "Dirichlet walls (ncl=2)", "beta": 0.259}
+    for istret in (0, 2):
+        p = X.make_params(256, 129, 128, xlx=4.0 * math.pi, yly=2.0, zlz=4.0 * math.pi / 3.0, ncly1=2, nclyn=2,
+                          istret=istret, beta=0.259)
+        X.init(p)
+        X.set_stream(stream.cuda_stream)
+        try:
+            for dt, lab in ((torch.float64, "f64"), (torch.float32, "f32")):
+                with torch.cuda.stream(stream):
+                    rhs = micro_field((128, 128, 256), True, dt, 20261203)
+                    rhs -= rhs.mean()
+                    X.poisson_async(rhs)
+                    ms = _events_time(stream, lambda: X.poisson_async(rhs), reps)
+                    out[f"istret{istret}_{lab}_ms"] = round(ms, 4)
+                    del rhs
+                    torch.cuda.synchronize()
+        finally:
+            X.finalize()
+    return out
+
+
 def learned_les_comparison(X, stream, fp64_ms_256, n_dns):
     """Time to solution at equal dissipation error, TGV Re=5000 to t=12:
     the FP64 plain solver on the 256^3 DNS grid against the FP32-bulk solver
@@ -373,6 +400,7 @@ def main():
         out["learned_les_equal_error"] = learned_les_comparison(X, stream, r64["event_ms_per_step"], n)
     if args.micro_n > 0:
         out["microbench"] = [microbench(X, args.micro_n, False, stream), microbench(X, args.micro_n, True, stream)]
+        out["stretched_poisson"] = stretched_poisson_bench(X, stream)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
         out["cpu_baseline"]["host"] = f"{os.cpu_count()} logical CPUs visible"
