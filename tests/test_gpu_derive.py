@@ -21,6 +21,9 @@ CONFIGS = [
     ("periodic", dict(nx=64, ny=48, nz=40)),
     ("periodic_small", dict(nx=16, ny=8, nz=24)),
     ("ywalls", dict(nx=32, ny=33, nz=16, ncly1=2, nclyn=2)),
+    # 513-point wall lines: blocks of 16 (FP64) / 32 (FP32) lines, so that a
+    # wavefront spans several segments of a line
+    ("ywalls_long", dict(nx=16, ny=513, nz=8, ncly1=2, nclyn=2)),
     ("freeslip", dict(nx=17, ny=19, nz=21, nclx1=1, nclxn=1, ncly1=1, nclyn=1, nclz1=1, nclzn=1)),
     ("mixed", dict(nx=18, ny=22, nz=20, nclx1=1, nclxn=2, ncly1=2, nclyn=1, nclz1=2, nclzn=2)),
 ]
