@@ -584,8 +584,10 @@ struct YzShape {
 // ---------------------------------------------------------------- kernel
 // AX: 0 x (LDS-staged), 1 y, 2 z (direct); EPI: fused product input and
 // epilogue present (a plain derivative compiles without them).
-template <typename T, int OP, int L, int AX, bool PERIODIC, bool EPI>
-__global__ void __launch_bounds__(1024) line_kernel(const LineArgs<T> A) {
+// LB: the largest block the instantiation is launched with (the register
+// budget follows from it).
+template <typename T, int OP, int L, int AX, bool PERIODIC, bool EPI, int LB = 1024>
+__global__ void __launch_bounds__(LB) line_kernel(const LineArgs<T> A) {
   constexpr bool XMODE = AX == 0;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
@@ -1046,6 +1048,23 @@ template <typename T, int OP, int L>
 static int launch_l(const LineArgs<T> &a, int ax, bool periodic, bool epi, int TL, int P, dim3 grid,
                     size_t shmem, hipStream_t st) {
   dim3 block(TL, P);
+  // FP64 64-point segments of y/z wall lines up to 640 points (e.g. 513):
+  // a 640-thread register budget (170 VGPRs instead of 128) keeps the
+  // 72-point windows out of scratch (FP32 fits either way)
+  if constexpr (L == 64 && sizeof(T) == 8) {
+    if (ax != 0 && !periodic && TL * P <= 640) {
+      if (ax == 1) {
+        if (epi) hipLaunchKernelGGL((line_kernel<T, OP, L, 1, false, true, 640>), grid, block, shmem, st, a);
+        else hipLaunchKernelGGL((line_kernel<T, OP, L, 1, false, false, 640>), grid, block, shmem, st, a);
+      } else {
+        if (epi) hipLaunchKernelGGL((line_kernel<T, OP, L, 2, false, true, 640>), grid, block, shmem, st, a);
+        else hipLaunchKernelGGL((line_kernel<T, OP, L, 2, false, false, 640>), grid, block, shmem, st, a);
+      }
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("line kernel launch: ") + hipGetErrorString(e));
+      return X3D_OK;
+    }
+  }
   if (ax == 0) launch_ax<T, OP, L, 0>(a, periodic, epi, grid, block, shmem, st);
   else if (ax == 1) launch_ax<T, OP, L, 1>(a, periodic, epi, grid, block, shmem, st);
   else launch_ax<T, OP, L, 2>(a, periodic, epi, grid, block, shmem, st);
