@@ -70,8 +70,9 @@ struct PoissonPlan {
   PoissonTables<double> t64;
   PoissonTables<float> t32;
   void *cw64 = nullptr, *cw32 = nullptr;   // spectral work arrays
-  // stretched y (poisson_010 with istret = 1, 2): factors of the pentadiagonal
-  // systems, five FP64 arrays [kz][parity][row][kx] (see build_penta_factors)
+  // stretched y (poisson_010 with istret != 0): factors of the pentadiagonal
+  // systems, five FP64 arrays [kz][system][row][kx] (see build_penta_factors);
+  // ystret = the number of systems per (kx, kz): 2 (istret 1, 2), 1 (istret 3)
   int ystret = 0;
   double *pf = nullptr;
   size_t pf_stride = 0;
@@ -756,11 +757,12 @@ static int build_tables(const PoissonPlan &P, const std::vector<double> *tabs, v
 }
 
 // ---------------------------------------------------------------- stretched y
-// poisson_010 with istret = 1, 2 (poisson.f90:846-879): after the x, y
+// poisson_010 with istret != 0 (poisson.f90:846-895): after the x, y
 // (cosine) and z transforms, the even cosine modes 0, 2, 4, ... and the odd
 // modes 1, 3, 5, ... of each (kx, kz) line solve the pentadiagonal systems of
 // matrice_refinement (poisson.f90:1814-2038) with inversion5_v1
-// (tools.f90:1225-1362).  Their matrices are real (the reference's real and
+// (tools.f90:1225-1362); with istret = 3 all modes solve one system
+// (poisson.f90:2126-2247, inversion5_v2 at tools.f90:1365-1497).  Their matrices are real (the reference's real and
 // imaginary parts are equal for periodic x and z) and do not depend on the
 // right-hand side, so the elimination runs once here, on the host in FP64, in
 // the reference's order, and the kernel applies the stored factors:
@@ -777,7 +779,11 @@ static void build_penta_factors(const Context &c, const PoissonPlan &P, const st
                                 std::vector<double> &out) {
   const DirCoef &Y = c.dir[1];
   const double pi = std::acos(-1.0);
-  const int nym = P.ny, nh = nym / 2, nxh = P.nxh, nz = P.nz;
+  // istret = 3: one system over all nym modes (poisson.f90:2126-2247,
+  // inversion5_v2), else the even and odd modes apart
+  const bool one = c.p.istret == 3;
+  const int nsys = one ? 1 : 2;
+  const int nym = P.ny, nh = nym / nsys, nxh = P.nxh, nz = P.nz;
   const double eps = 1.0e-16;
   // yky = nym wp without 1/yly on a stretched mesh (poisson.f90:1614)
   std::vector<double> yky(nym, 0.0);
@@ -789,61 +795,80 @@ static void build_penta_factors(const Context &c, const PoissonPlan &P, const st
   }
   const double alpha = c.alpha, beta = c.p.beta;
   const double xa0 = alpha / pi + 0.5 / beta / pi;
-  const double xa1 = c.p.istret == 1 ? 1.0 / 4.0 / beta / pi : -(1.0 / 4.0 / beta / pi);
+  const double xa1 = c.p.istret == 1 ? 1.0 / 4.0 / beta / pi : -(1.0 / 4.0 / beta / pi);   // 2 and 3: -
   const double xa0_2 = xa0 * xa0, xa1_2 = xa1 * xa1, xa01 = xa0 * xa1, xa0p1_2 = (xa0 + xa1) * (xa0 + xa1);
-  const size_t stride = (size_t)nz * 2 * nh * nxh;
+  const size_t stride = (size_t)nz * nsys * nh * nxh;
   out.assign(5 * stride, 0.0);
   std::vector<double> cw(nh), a(5 * nh);
   for (int kz = 0; kz < nz; ++kz)
-    for (int par = 0; par < 2; ++par)
+    for (int par = 0; par < nsys; ++par)
       for (int kx = 0; kx < nxh; ++kx) {
         const double TX = tx[kx], TZ = tz[kz], TX2 = TX * TX, TZ2 = TZ * TZ, XK2 = kx2[kx], ZK2 = kz2[kz];
-        for (int j = 0; j < nh; ++j) cw[j] = TX * (yky[2 * j + par] * TZ);
+        for (int j = 0; j < nh; ++j) cw[j] = TX * (yky[nsys * j + par] * TZ);
         auto A = [&](int j, int d) -> double & { return a[5 * j + d]; };
         std::fill(a.begin(), a.end(), 0.0);
         auto kxz = [&](int mode) { const double t2 = ty[mode] * ty[mode]; return XK2 * t2 * TZ2 + ZK2 * t2 * TX2; };
         const int n = nh - 1;
-        // main diagonal
-        for (int j = 1; j < n; ++j)
-          A(j, 2) = -(kxz(2 * j + par) + xa0_2 * (cw[j] * cw[j]) + xa1_2 * cw[j] * (cw[j - 1] + cw[j + 1]));
-        if (par == 0) {
+        if (one) {
+          // istret = 3 (poisson.f90:2148-2233): the interior formulas on
+          // every row, the first row's off-diagonals without the factor 2
+          // of the even system of istret 1, 2
+          for (int j = 1; j < n; ++j)
+            A(j, 2) = -(kxz(j) + xa0_2 * (cw[j] * cw[j]) + xa1_2 * cw[j] * (cw[j - 1] + cw[j + 1]));
           A(0, 2) = -(kxz(0) + xa0_2 * (cw[0] * cw[0]) + xa1_2 * cw[0] * cw[1]);
-          A(n, 2) = -(kxz(2 * n) + xa0_2 * (cw[n] * cw[n]) + xa1_2 * cw[n] * cw[n - 1]);
+          A(n, 2) = -(kxz(n) + xa0_2 * (cw[n] * cw[n]) + xa1_2 * cw[n] * cw[n - 1]);
+          for (int j = 0; j < n; ++j) A(j, 3) = xa01 * (cw[j + 1] * (cw[j] + cw[j + 1]));
+          for (int j = 0; j + 2 < nh; ++j) A(j, 4) = -xa1_2 * (cw[j + 1] * cw[j + 2]);
+          for (int j = 1; j < nh; ++j) A(j, 1) = xa01 * (cw[j - 1] * (cw[j] + cw[j - 1]));
+          for (int j = 2; j < nh; ++j) A(j, 0) = -xa1_2 * (cw[j - 1] * cw[j - 2]);
+          if (XK2 == 0.0 && ZK2 == 0.0) {
+            A(0, 2) = 1.0;
+            A(0, 3) = 0.0;
+            A(0, 4) = 0.0;
+          }
         } else {
-          A(0, 2) = -(kxz(1) + (xa0_2 - xa1_2) * (cw[0] * cw[0]) + xa1_2 * cw[0] * cw[1]);
-          A(n, 2) = -(kxz(2 * n + 1) + xa0p1_2 * (cw[n] * cw[n]) + xa1_2 * cw[n] * cw[n - 1]);
-        }
-        // +1
-        for (int j = 1; j < n; ++j) A(j, 3) = xa01 * (cw[j + 1] * (cw[j] + cw[j + 1]));
-        if (par == 0) {
-          A(0, 3) = 2.0 * xa01 * (cw[0] * cw[1] + cw[1] * cw[1]);
-        } else {
-          A(0, 3) = (xa0 - xa1) * xa1 * (cw[0] * cw[1]) + xa0 * xa1 * (cw[1] * cw[1]);
-          A(n - 1, 3) = xa0 * xa1 * cw[n - 1] * cw[n] + (xa0 + xa1) * xa1 * (cw[n] * cw[n]);
-          A(n, 3) = 0.0;
-        }
-        // +2
-        for (int j = 0; j < nh - 2; ++j) A(j, 4) = xa1_2 * (-cw[j + 1] * cw[j + 2]);
-        if (par == 0) A(0, 4) = 2.0 * A(0, 4);
-        A(n - 1, 4) = 0.0;
-        A(n, 4) = 0.0;
-        // -1
-        for (int j = 1; j < nh; ++j) A(j, 1) = xa01 * (cw[j - 1] * (cw[j] + cw[j - 1]));
-        A(0, 1) = 0.0;
-        if (par == 1) {
-          A(1, 1) = xa0 * xa1 * (cw[1] * cw[0]) + (xa0 + xa1) * xa1 * (cw[0] * cw[0]);
-          A(n, 1) = (xa0 + xa1) * xa1 * (cw[n] * cw[n - 1]) + xa0 * xa1 * (cw[n - 1] * cw[n - 1]);
-        }
-        // -2
-        for (int j = 2; j < nh; ++j) A(j, 0) = xa1_2 * (-cw[j - 1] * cw[j - 2]);
-        // the mean mode: unit first row of the even system
-        if (par == 0 && XK2 == 0.0 && ZK2 == 0.0) {
-          A(0, 2) = 1.0;
-          A(0, 3) = 0.0;
-          A(0, 4) = 0.0;
+          // main diagonal
+          for (int j = 1; j < n; ++j)
+            A(j, 2) = -(kxz(2 * j + par) + xa0_2 * (cw[j] * cw[j]) + xa1_2 * cw[j] * (cw[j - 1] + cw[j + 1]));
+          if (par == 0) {
+            A(0, 2) = -(kxz(0) + xa0_2 * (cw[0] * cw[0]) + xa1_2 * cw[0] * cw[1]);
+            A(n, 2) = -(kxz(2 * n) + xa0_2 * (cw[n] * cw[n]) + xa1_2 * cw[n] * cw[n - 1]);
+          } else {
+            A(0, 2) = -(kxz(1) + (xa0_2 - xa1_2) * (cw[0] * cw[0]) + xa1_2 * cw[0] * cw[1]);
+            A(n, 2) = -(kxz(2 * n + 1) + xa0p1_2 * (cw[n] * cw[n]) + xa1_2 * cw[n] * cw[n - 1]);
+          }
+          // +1
+          for (int j = 1; j < n; ++j) A(j, 3) = xa01 * (cw[j + 1] * (cw[j] + cw[j + 1]));
+          if (par == 0) {
+            A(0, 3) = 2.0 * xa01 * (cw[0] * cw[1] + cw[1] * cw[1]);
+          } else {
+            A(0, 3) = (xa0 - xa1) * xa1 * (cw[0] * cw[1]) + xa0 * xa1 * (cw[1] * cw[1]);
+            A(n - 1, 3) = xa0 * xa1 * cw[n - 1] * cw[n] + (xa0 + xa1) * xa1 * (cw[n] * cw[n]);
+            A(n, 3) = 0.0;
+          }
+          // +2
+          for (int j = 0; j < nh - 2; ++j) A(j, 4) = xa1_2 * (-cw[j + 1] * cw[j + 2]);
+          if (par == 0) A(0, 4) = 2.0 * A(0, 4);
+          A(n - 1, 4) = 0.0;
+          A(n, 4) = 0.0;
+          // -1
+          for (int j = 1; j < nh; ++j) A(j, 1) = xa01 * (cw[j - 1] * (cw[j] + cw[j - 1]));
+          A(0, 1) = 0.0;
+          if (par == 1) {
+            A(1, 1) = xa0 * xa1 * (cw[1] * cw[0]) + (xa0 + xa1) * xa1 * (cw[0] * cw[0]);
+            A(n, 1) = (xa0 + xa1) * xa1 * (cw[n] * cw[n - 1]) + xa0 * xa1 * (cw[n - 1] * cw[n - 1]);
+          }
+          // -2
+          for (int j = 2; j < nh; ++j) A(j, 0) = xa1_2 * (-cw[j - 1] * cw[j - 2]);
+          // the mean mode: unit first row of the even system
+          if (par == 0 && XK2 == 0.0 && ZK2 == 0.0) {
+            A(0, 2) = 1.0;
+            A(0, 3) = 0.0;
+            A(0, 4) = 0.0;
+          }
         }
         // inversion5_v1's elimination, recording the factors
-        const size_t line = ((size_t)(kz * 2 + par) * nh) * nxh + kx;
+        const size_t line = ((size_t)(kz * nsys + par) * nh) * nxh + kx;
         auto F = [&](int q, int m) -> double & { return out[q * stride + line + (size_t)m * nxh]; };
         for (int m = 0; m + 2 < nh; ++m) {
           const double piv = A(m, 2);
@@ -875,21 +900,22 @@ static void build_penta_factors(const Context &c, const PoissonPlan &P, const st
       }
 }
 
-// One thread per (kx, kz, parity): forward elimination with the stored
+// One thread per (kx, kz, system): forward elimination with the stored
 // multipliers (the scaled right-hand side read once, the eliminated values
 // written back), then the last 2x2 block and the back substitution.  The
 // (nx/2+1, :, nz/2+1) line is zeroed (poisson.f90:902-908).
 template <typename T>
 __global__ void __launch_bounds__(64) penta_y_kernel(cplx<T> *__restrict__ cw, const double *__restrict__ pf,
-                                                     size_t stride, int nxh, int ny, int nz, double scale) {
+                                                     size_t stride, int nxh, int ny, int nz, int nsys,
+                                                     double scale) {
   const int kx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int kz = blockIdx.y >> 1, par = blockIdx.y & 1;
+  const int kz = blockIdx.y / nsys, par = blockIdx.y - kz * nsys;
   if (kx >= nxh) return;
-  const int nh = ny >> 1, n = nh - 1;
+  const int nh = ny / nsys, n = nh - 1;
   const size_t line = ((size_t)blockIdx.y * nh) * nxh + kx;
   const double *F1 = pf + line, *F2 = pf + stride + line, *RP = pf + 2 * stride + line;
   const double *A1 = pf + 3 * stride + line, *B1 = pf + 4 * stride + line;
-  auto at = [&](int m) -> cplx<T> & { return cw[((size_t)kz * ny + (2 * m + par)) * nxh + kx]; };
+  auto at = [&](int m) -> cplx<T> & { return cw[((size_t)kz * ny + (nsys * m + par)) * nxh + kx]; };
   if (kx == nxh - 1 && kz == (nz >> 1)) {
     for (int m = 0; m < nh; ++m) at(m) = cplx<T>{T(0), T(0)};
     return;
@@ -967,13 +993,13 @@ int poisson_setup(Context &c) {
   if (c.p.istret != 0) {
     // stretched y: poisson_010's pentadiagonal branch (istret = 3 uses
     // inversion5_v2 on the whole line, not provided)
-    if (P->xdct || P->zdct || !P->ydct || (c.p.istret != 1 && c.p.istret != 2)) {
+    if (P->xdct || P->zdct || !P->ydct) {
       delete P;
-      return fail(X3D_ERR_UNSUPPORTED, "Poisson on a stretched y mesh: poisson_010 with istret = 1 or 2 only");
+      return fail(X3D_ERR_UNSUPPORTED, "Poisson on a stretched y mesh: poisson_010 (x and z periodic) only");
     }
     std::vector<double> f;
     build_penta_factors(c, *P, tabs[0], tabs[1], tabs[3], tabs[4], tabs[5], f);
-    P->ystret = 1;
+    P->ystret = c.p.istret == 3 ? 1 : 2;   // number of systems per (kx, kz)
     P->pf_stride = f.size() / 5;
     hipError_t e = hipMalloc(&P->pf, f.size() * sizeof(double));
     if (e == hipSuccess) e = hipMemcpy(P->pf, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice);
@@ -1093,9 +1119,9 @@ int poisson_solve(Context &c, T *rhs, hipStream_t st) {
     // stretched y: z transform, pentadiagonal solves along y, inverse z
     ok = ok && dispatch_s<T, FFT_Z_FWD>(P->nz, *P, cw, tb, st);
     if (ok) {
-      const dim3 grid((P->nxh + 63) / 64, 2 * P->nz), block(64);
+      const dim3 grid((P->nxh + 63) / 64, P->ystret * P->nz), block(64);
       hipLaunchKernelGGL((penta_y_kernel<T>), grid, block, 0, st, cw, P->pf, P->pf_stride, P->nxh, P->ny, P->nz,
-                         P->scale);
+                         P->ystret, P->scale);
     }
     ok = ok && dispatch_s<T, FFT_Z_INV>(P->nz, *P, cw, tb, st);
   } else {

@@ -943,8 +943,8 @@ def matrice_refinement(S: Schemes):
     (nz/2+1, nym/2, nx, 5) over (kz, row, kx, diagonal -2..+2).  Entries are
     real: the reference's real and imaginary parts are equal here."""
     p = S.p
-    if p.istret not in (1, 2):
-        raise NotImplementedError("matrice_refinement: istret = 3 (inversion5_v2) is not restated")
+    if p.istret == 3:
+        return (_matrice_refinement_3(S),)
     alpha, beta = S.stret["alpha"], p.beta
     tx, ty, tz, kx2, kz2, yky = _refinement_1d(S)
     nyh = p.ny // 2                      # the velocity grid's ny/2 = nym/2
@@ -1011,6 +1011,48 @@ def matrice_refinement(S: Schemes):
     a[:, 0, :, 3] = np.where(sing, 0.0, a[:, 0, :, 3])
     a[:, 0, :, 4] = np.where(sing, 0.0, a[:, 0, :, 4])
     return a, a2
+
+
+def _matrice_refinement_3(S: Schemes):
+    """istret = 3 (poisson.f90:2126-2247): one pentadiagonal system over all
+    nym cosine modes, cw22(j) = transx * yky(j) * transz, shape
+    (nz/2+1, nym, nx, 5)."""
+    p = S.p
+    alpha, beta = S.stret["alpha"], p.beta
+    tx, ty, tz, kx2, kz2, yky = _refinement_1d(S)
+    nym = p.nym
+    xa0 = alpha / PI + 0.5 / beta / PI
+    xa1 = -(1.0 / 4.0 / beta / PI)
+    xa0_2, xa1_2, xa01 = xa0 * xa0, xa1 * xa1, xa0 * xa1
+    TX = tx[None, :]; TZ = tz[:, None]
+    TX2 = TX ** 2; TZ2 = TZ ** 2
+    XK2 = kx2[None, :]; ZK2 = kz2[:, None]
+    ty2 = ty ** 2
+    cw = [TX * (yky[j] * TZ) for j in range(nym)]
+    a = np.zeros((tz.size, nym, tx.size, 5))
+
+    def kxz(t2):
+        return XK2 * t2 * TZ2 + ZK2 * t2 * TX2
+
+    n = nym - 1
+    for j in range(1, n):
+        a[:, j, :, 2] = -(kxz(ty2[j]) + xa0_2 * cw[j] ** 2 + xa1_2 * cw[j] * (cw[j - 1] + cw[j + 1]))
+    a[:, 0, :, 2] = -(kxz(ty2[0]) + xa0_2 * cw[0] ** 2 + xa1_2 * cw[0] * cw[1])
+    a[:, n, :, 2] = -(kxz(ty2[n]) + xa0_2 * cw[n] ** 2 + xa1_2 * cw[n] * cw[n - 1])
+    for j in range(0, n):
+        a[:, j, :, 3] = xa01 * (cw[j + 1] * (cw[j] + cw[j + 1]))
+    for j in range(0, nym - 2):
+        a[:, j, :, 4] = -xa1_2 * (cw[j + 1] * cw[j + 2])
+    for j in range(1, nym):
+        a[:, j, :, 1] = xa01 * (cw[j - 1] * (cw[j] + cw[j - 1]))
+    for j in range(2, nym):
+        a[:, j, :, 0] = -xa1_2 * (cw[j - 1] * cw[j - 2])
+    # the mean mode (kx = kz = 0): unit first row (:2240-2245)
+    sing = (XK2 == 0.0) & (ZK2 == 0.0)
+    a[:, 0, :, 2] = np.where(sing, 1.0, a[:, 0, :, 2])
+    a[:, 0, :, 3] = np.where(sing, 0.0, a[:, 0, :, 3])
+    a[:, 0, :, 4] = np.where(sing, 0.0, a[:, 0, :, 4])
+    return a
 
 
 EPS_INV5 = 1.0e-16   # tools.f90:1241-1245 (DOUBLE_PREC)
@@ -1108,10 +1150,16 @@ def poisson_010(S: Schemes, rhs):
     else:
         # stretched y: even and odd cosine modes each solve a pentadiagonal
         # system, real and imaginary parts separately (poisson.f90:846-879)
-        a, a2 = matrice_refinement(S)
-        for par, A in ((0, a), (1, a2)):
-            c1[:, par::2, :] = inversion5_v1(A, c1[:, par::2, :])
-            c2[:, par::2, :] = inversion5_v1(A, c2[:, par::2, :])
+        mats = matrice_refinement(S)
+        if len(mats) == 1:
+            # istret = 3: one system over all modes, inversion5_v2
+            # (tools.f90:1365-1497, the same elimination on nym rows)
+            c1 = inversion5_v1(mats[0], c1)
+            c2 = inversion5_v1(mats[0], c2)
+        else:
+            for par, A in ((0, mats[0]), (1, mats[1])):
+                c1[:, par::2, :] = inversion5_v1(A, c1[:, par::2, :])
+                c2[:, par::2, :] = inversion5_v1(A, c2[:, par::2, :])
     # zero the (nx/2+1, :, nz/2+1) line (poisson.f90:902-908)
     if nz // 2 < nzh:
         c1[nz // 2, :, nx // 2] = 0.0

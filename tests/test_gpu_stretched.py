@@ -1,8 +1,8 @@
-"""GPU parity on a stretched y mesh (istret = 1, 2; stretching.f90) against
+"""GPU parity on a stretched y mesh (istret = 1, 2, 3; stretching.f90) against
 the oracle, through the C-ABI: the metrics x3d_init computes, the y
 derivatives that multiply by them (dery * ppy, deryvp * ppyi, derypv * ppy),
-and the stretched poisson_010 (matrice_refinement + inversion5_v1 applied as
-stored pentadiagonal factors between the z transforms).
+and the stretched poisson_010 (matrice_refinement + inversion5_v1 / _v2
+applied as stored pentadiagonal factors between the z transforms).
 
 Bar: 1e-12 relative in FP64; 2e-5 (derivatives) and 1e-5 (Poisson) for the
 FP32 entries.  deryy on a stretched mesh is not provided and must say so.
@@ -19,6 +19,7 @@ CASES = [
     ("s2_walls_32x33x16", dict(nx=32, ny=33, nz=16, ncly1=2, nclyn=2, istret=2, beta=0.259)),
     ("s1_freeslip_64x65x32", dict(nx=64, ny=65, nz=32, ncly1=1, nclyn=1, istret=1, beta=0.7)),
     ("s2_mixed_16x129x64", dict(nx=16, ny=129, nz=64, ncly1=2, nclyn=1, istret=2, beta=0.5)),
+    ("s3_half_32x65x16", dict(nx=32, ny=65, nz=16, ncly1=2, nclyn=1, istret=3, beta=0.4)),
 ]
 
 
@@ -112,6 +113,8 @@ def test_poisson_stretched_inverts_div_grad(ctx):
     import torch
     import scipy.fft as sf
     name, p, S, X = ctx
+    if p.istret == 3:
+        pytest.skip("istret = 3: the reference's first matrix row is not the exact operator's")
     pe = np.random.default_rng(9).standard_normal((p.nz, p.nym, p.nx))
     F = sf.dct(np.fft.fft(np.fft.fft(pe, axis=0), axis=2), type=2, axis=1)
     F[p.nz // 2, :, :] = 0.0

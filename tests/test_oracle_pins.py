@@ -296,6 +296,38 @@ def test_matrice_refinement_is_the_stretched_operator(istret):
     assert Ae[0, 0] == 1.0 and np.all(Ae[0, 1:] == 0.0)
 
 
+def test_matrice_refinement_istret3_rows():
+    """istret = 3 (one system over all modes): every row but the first
+    equals the stretched div(grad .) in cosine modes (here at kx = 1, where
+    the first row couples to other modes); the first row's off-diagonals are
+    half of the operator's, as the reference writes them without the factor
+    2 that its even system of istret 1, 2 carries (poisson.f90:2196-2198,
+    :2205-2207 against :2034-2035, :2059)."""
+    import scipy.fft as sf
+    p = O.Params(8, 17, 4, ncly1=1, nclyn=1, istret=3, beta=0.7)
+    S = O.Schemes(p)
+    nym = p.nym
+    x = np.arange(p.nx)
+    cx = np.cos(2.0 * np.pi * x / p.nx)
+    M = np.zeros((nym, nym))
+    for m in range(nym):
+        e = np.zeros(nym)
+        e[m] = 1.0
+        f = (cx[None, None, :] * sf.idct(e, type=2)[None, :, None]) * np.ones((p.nz, 1, 1))
+        r = O.divergence(S, *O.gradp(S, f))
+        M[:, m] = sf.dct((r[0] * cx[None, :]).sum(axis=1) / (p.nx / 2), type=2)
+    (a,) = O.matrice_refinement(S)
+    A = np.zeros((nym, nym))
+    for j in range(nym):
+        for d in range(5):
+            if 0 <= j + d - 2 < nym:
+                A[j, j + d - 2] = a[0, j, 1, d]
+    tol = 1e-11 * np.abs(M).max()
+    assert np.abs(A[1:] - M[1:]).max() < tol
+    assert abs(A[0, 0] - M[0, 0]) < tol
+    assert np.abs(A[0, 1:3] - 0.5 * M[0, 1:3]).max() < tol and np.abs(M[0, 1:3]).min() > 1e-3
+
+
 def _no_nyquist(p, f):
     import scipy.fft as sf
     F = sf.dct(np.fft.fft(np.fft.fft(f, axis=0), axis=2), type=2, axis=1)
