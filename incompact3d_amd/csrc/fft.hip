@@ -285,7 +285,12 @@ struct SShape {
   static constexpr int TL0 = (FFT_THREADS / NT) * (sizeof(T) == 4 ? 2 : 1);
   static constexpr int TL = TL0 < 64 ? TL0 : 64;
   static constexpr int threads(bool extra) { return (TL + (extra ? 1 : 0)) * NT; }
-  static constexpr size_t lds(bool extra) { return sizeof(cplx<T>) * ((size_t)(TL + (extra ? 1 : 0)) * N + 2 * N); }
+  // ztab: the z pass's tz and kz2 tables in FP64 (lines up to 256 points,
+  // where the 4 KB keep two blocks per CU)
+  static constexpr bool ZTAB = N <= 256;
+  static constexpr size_t lds(bool extra, bool ztab) {
+    return sizeof(cplx<T>) * ((size_t)(TL + (extra ? 1 : 0)) * N + 2 * N) + (ztab ? 2 * N * sizeof(double) : 0);
+  }
 };
 
 // row of point p under the even/odd reorder of poisson.f90:688-697 / :997-1008
@@ -457,6 +462,16 @@ __global__ void __launch_bounds__((SShape<T, N>::threads(EXTRA)))
   // the cosine transform of this pass: y in the y passes, z in the z pass
   const bool dct_here = ZL ? (MODE == FFT_Z_SOLVE && COS && zdct != 0) : ydct != 0;
   if (dct_here) stage_table(dct, ZL ? tb.dctz : tb.dcty, N);
+  // the z pass reads tz and kz2 at its 16 points per thread: staged in LDS
+  // with the twiddles instead of read from global memory after the FFT
+  constexpr bool ZT = MODE == FFT_Z_SOLVE && !COS && SShape<T, N>::ZTAB;
+  double *ztab = reinterpret_cast<double *>(dct + N);
+  if constexpr (ZT) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      ztab[i] = tb.tz[i];
+      ztab[N + i] = tb.kz2[i];
+    }
+  }
   const bool reorder_in = (MODE == FFT_Y_FWD || MODE == FFT_Z_SOLVE) && dct_here;
   cplx<T> x[E];
 #pragma unroll
@@ -554,9 +569,9 @@ __global__ void __launch_bounds__((SShape<T, N>::threads(EXTRA)))
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int kz = t + m * NT;
-        const double tzv = tb.tz[kz];
+        const double tzv = ZT ? ztab[kz] : tb.tz[kz];
         const double yz = tyv * tzv, xz = txv * tzv, xy = txv * tyv;
-        const double K = kx2 * (yz * yz) + ky2 * (xz * xz) + tb.kz2[kz] * (xy * xy);
+        const double K = kx2 * (yz * yz) + ky2 * (xz * xz) + (ZT ? ztab[N + kz] : tb.kz2[kz]) * (xy * xy);
         const bool zero = (K < 1.0e-16) || (zero_line && a == (nxh - 1) && kz == (N >> 1));
         const double f = zero ? 0.0 : -scale / K;
         x[m] = cplx<T>{(T)((double)x[m].x * f), (T)((double)x[m].y * f)};
@@ -1049,7 +1064,8 @@ static void launch_s1(const PoissonPlan &P, cplx<T> *cw, const PoissonTables<T> 
   const dim3 grid(ncols, ZPass<MODE>::value ? P.ny : P.nz), block(S::threads(EXTRA));
   // poisson_010 zeroes the (nx/2+1, :, nz/2+1) line (poisson.f90:902-908); poisson_11x does not
   const int zero_line = MODE == FFT_Z_SOLVE && P.ydct && !P.xdct;
-  hipLaunchKernelGGL((strided_kernel<T, N, MODE, COS, EXTRA>), grid, block, S::lds(EXTRA), st, cw, nmain, P.nxh,
+  const bool zt = MODE == FFT_Z_SOLVE && !COS && S::ZTAB;
+  hipLaunchKernelGGL((strided_kernel<T, N, MODE, COS, EXTRA>), grid, block, S::lds(EXTRA, zt), st, cw, nmain, P.nxh,
                      P.ny, P.nz, P.ydct, P.scale, zero_line, P.xdct, P.zdct, tb);
 }
 
