@@ -140,11 +140,9 @@ int der_entry(int dir, int op, int bc1, int bcn, T *t, const T *u, const T *ff, 
   L.bc1 = bc1;
   L.bcn = bcn;
   L.npaire = *npaire;
-  if (dir == 1 && c->p.istret != 0) {
-    if (op != OP_DER1)
-      return fail(X3D_ERR_UNSUPPORTED, "stretched y: deryy (pp2y/pp4y terms) is not provided");
-    X3D_TRY(y_metric<T>(*c, false, pp, n, &L.pp));
-  }
+  // dery multiplies by ppy; deryy does not (its caller combines pp2y and
+  // pp4y, transeq.f90:337-346)
+  if (dir == 1 && op == OP_DER1 && c->p.istret != 0) X3D_TRY(y_metric<T>(*c, false, pp, n, &L.pp));
   X3D_TRY(launch_line<T>(L, c->stream));
   X3D_HIP(hipStreamSynchronize(c->stream));
   return X3D_OK;

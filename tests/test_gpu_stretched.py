@@ -5,7 +5,7 @@ and the stretched poisson_010 (matrice_refinement + inversion5_v1 / _v2
 applied as stored pentadiagonal factors between the z transforms).
 
 Bar: 1e-12 relative in FP64; 2e-5 (derivatives) and 1e-5 (Poisson) for the
-FP32 entries.  deryy on a stretched mesh is not provided and must say so.
+FP32 entries.  deryy applies no metric, as in the reference.
 """
 import numpy as np
 import pytest
@@ -127,13 +127,20 @@ def test_poisson_stretched_inverts_div_grad(ctx):
     assert np.abs((pr - pr.mean()) - (pe - pe.mean())).max() <= 1e-10 * np.abs(pe).max(), name
 
 
-def test_deryy_stretched_is_refused(ctx):
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_deryy_stretched_is_plain(ctx, dtype):
+    """deryy applies no metric on a stretched mesh (its caller combines
+    pp2y and pp4y, transeq.f90:337-346): the plain second derivative."""
     import torch
     from incompact3d_amd import derive as D
     name, p, S, X = ctx
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    u = seeded_field((p.nz, p.ny, p.nx), seed=23, lengths=(p.xlx, p.yly, p.zlz))
+    ref = O.der2(S, u, "y", 0)
     c = S.d2["y"]
-    u = torch.zeros((p.nz, p.ny, p.nx), dtype=torch.float64, device="cuda")
-    t = torch.empty_like(u)
+    ud = torch.tensor(u, dtype=tdt, device="cuda")
+    td = torch.empty_like(ud)
     fn = getattr(D, f"deryy_{S.bc['y'][0]}{S.bc['y'][1]}")
-    with pytest.raises(X.X3DError, match="stretched"):
-        fn(t, u, None, None, c["sf"], c["ss"], c["sw"], p.nx, p.ny, p.nz, 0)
+    fn(td, ud, None, None, c["sf"], c["ss"], c["sw"], p.nx, p.ny, p.nz, 0)
+    err = rel_err(td.double().cpu().numpy(), ref)
+    assert err <= _tol(dtype, 2e-5), f"{name}: {err:.3e}"
