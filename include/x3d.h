@@ -72,8 +72,9 @@ int x3d_device_name(char *buf, int len);
 /* Copy a coefficient array computed by x3d_init into host memory, so that a
  * caller can hand the same arrays back (the reference's module arrays).
  * name is the reference variable name: "ffx","fsx","fwx","ffxp",...,
- * "sfx",...,"cfx6","csx6","cwx6","cifxp6",... ("y"/"z" variants likewise).
- * Returns the array length in *n. */
+ * "sfx",...,"cfx6","csx6","cwx6","cifxp6",... ("y"/"z" variants likewise),
+ * and with istret != 0 the stretching.f90 arrays "yp","ypi","ppy","ppyi"
+ * (stretching.f90:92-318).  Returns the array length in *n. */
 int x3d_get_coefficients(const char *name, double *out, int cap, int *n);
 /* Scalar coefficient by reference name: "afix","bfix","alfaix","asix",...,
  * "acix6","alcaix6","aicix6","ailcaix6", "xnu", "dx", ... */
