@@ -315,6 +315,7 @@ int x3d_schemes_host(const x3d_params *p, const char *name, double *out, int cap
   if (c.p.ifirstder == 0) c.p.ifirstder = 4;
   if (c.p.isecondder == 0) c.p.isecondder = 4;
   X3D_TRY(build_schemes(c));
+  X3D_TRY(compute_stretching(c));
   const std::string s = name;
   for (int d = 0; d < 3; ++d)
     for (const auto &kv : c.dir[d].scalars)

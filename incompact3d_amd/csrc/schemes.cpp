@@ -515,7 +515,7 @@ template int make_line_call<float>(const Context &, int, int, int, LineCall<floa
 // yeta/yetai are the computational coordinates of the velocity and pressure
 // points, yp/ypi the physical ones, ppy/ppyi = yly (alpha/pi + sin^2(pi
 // eta)/(pi beta)) the metric that the y derivatives multiply by.
-int build_stretching(Context &c) {
+int compute_stretching(Context &c) {
   const x3d_params &p = c.p;
   if (p.istret == 0) return X3D_OK;
   if (p.istret < 0 || p.istret > 3) return fail(X3D_ERR_ARG, "istret must be 0, 1, 2 or 3");
@@ -565,6 +565,13 @@ int build_stretching(Context &c) {
     c.ppy[j] = yly * (alpha / pi + (1.0 / pi / beta) * s1 * s1);
     c.ppyi[j] = yly * (alpha / pi + (1.0 / pi / beta) * s2 * s2);
   }
+  return X3D_OK;
+}
+
+int build_stretching(Context &c) {
+  if (c.p.istret == 0) return X3D_OK;
+  X3D_TRY(compute_stretching(c));
+  const int ny = c.p.ny;
   std::vector<float> f(ny);
   X3D_HIP(hipMalloc(&c.ppy64, ny * sizeof(double)));
   X3D_HIP(hipMalloc(&c.ppyi64, ny * sizeof(double)));

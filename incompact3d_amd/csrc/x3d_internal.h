@@ -122,7 +122,8 @@ int require_ctx(Context **c);
 
 // schemes.cpp
 int build_schemes(Context &c);
-int build_stretching(Context &c);   // istret != 0: metrics on host and device
+int compute_stretching(Context &c); // istret != 0: the metrics on the host
+int build_stretching(Context &c);   // ... and their device copies
 void free_stretching(Context &c);
 int upload_sets(Context &c);
 void free_sets(Context &c);

@@ -86,6 +86,21 @@ def test_host_schemes_equal_oracle(lib, kw):
             assert X.schemes_host(xp, name) == ref
 
 
+@pytest.mark.parametrize("istret,beta", [(1, 0.7), (2, 0.259), (3, 0.4)])
+def test_host_stretching_equals_oracle(lib, istret, beta):
+    """The C++ restatement of stretching.f90 and the oracle's, on the host:
+    yp, ypi, ppy, ppyi agree to the last few ulps (libm and numpy may round
+    sin/tan/atan differently by an ulp)."""
+    import incompact3d_amd as X
+    kw = dict(nx=16, ny=65, nz=8, ncly1=2, nclyn=1, istret=istret, beta=beta)
+    st = O.stretching(O.Params(**kw))
+    xp = X.make_params(**kw)
+    for name in ("yp", "ypi", "ppy", "ppyi"):
+        got, ref = X.schemes_host(xp, name), st[name]
+        assert got.shape == ref.shape
+        assert np.abs(got - ref).max() <= 4e-16 * np.abs(ref).max(), name
+
+
 def test_entry_points_fail_cleanly_without_context(lib):
     n = C.c_int(8)
     rc = lib.x3d_poisson(None, C.byref(n), C.byref(n), C.byref(n))
