@@ -1,5 +1,6 @@
 // extern "C" boundary of libx3d: context management and the Fortran-mirror
 // entries of derive.f90 / poisson.f90 / 2DECOMP transposes.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -17,6 +18,14 @@ std::mutex g_ctx_mu;
 // coefficient sets handed in by a caller that are not one of the library's
 // own (uploaded on first use and kept)
 std::vector<TriSet> g_foreign;
+// metric arrays passed by callers that differ from x3d_init's (e.g. computed
+// by another libm): host values in the precision given, and their device copy
+struct ForeignMetric {
+  std::vector<double> v64;
+  std::vector<float> v32;
+  void *dev = nullptr;
+};
+std::vector<ForeignMetric> g_foreign_metric;
 }  // namespace
 
 void set_error(const std::string &m) { g_err = m; }
@@ -98,19 +107,40 @@ int check_dims(const int *a, const int *b, const int *c) {
 // derx/dery/derz/derxx/deryy/derzz_<bc1><bcn> (derive.f90:7-3795)
 // The metric a y derivative multiplies by on a stretched mesh (derive.f90:
 // 409-417 ppy for dery, :4572-4578 ppyi for deryvp, :4905-4911 ppy for
-// derypv): the context's device copy.  A caller's host array, when given, must
-// hold the values x3d_init computed (the stretching of the same parameters).
+// derypv), as the reference does with the array its caller passes: the
+// context's device copy when the values are x3d_init's (or none is passed),
+// else the caller's values, uploaded once and kept by content.
 template <typename T>
 int y_metric(const Context &c, bool pressure, const T *caller, int n, const T **dev) {
   *dev = nullptr;
   if (c.p.istret == 0) return X3D_OK;
   const std::vector<double> &m = pressure ? c.ppyi : c.ppy;
   if (n > (int)m.size()) return fail(X3D_ERR_ARG, "stretched y: line longer than the metric");
+  bool same = true;
   if (caller)
-    for (int j = 0; j < n; ++j)
-      if (caller[j] != (T)m[j]) return fail(X3D_ERR_ARG, "stretched y: the metric array differs from x3d_init's");
-  if constexpr (sizeof(T) == 8) *dev = (const T *)(pressure ? c.ppyi64 : c.ppy64);
-  else *dev = (const T *)(pressure ? c.ppyi32 : c.ppy32);
+    for (int j = 0; j < n && same; ++j) same = caller[j] == (T)m[j];
+  if (same) {
+    if constexpr (sizeof(T) == 8) *dev = (const T *)(pressure ? c.ppyi64 : c.ppy64);
+    else *dev = (const T *)(pressure ? c.ppyi32 : c.ppy32);
+    return X3D_OK;
+  }
+  for (const ForeignMetric &f : g_foreign_metric) {
+    bool hit;
+    if constexpr (sizeof(T) == 8) hit = f.v64.size() == (size_t)n && std::equal(f.v64.begin(), f.v64.end(), caller);
+    else hit = f.v32.size() == (size_t)n && std::equal(f.v32.begin(), f.v32.end(), caller);
+    if (hit) {
+      *dev = (const T *)f.dev;
+      return X3D_OK;
+    }
+  }
+  if (g_foreign_metric.size() >= 64) return fail(X3D_ERR_ARG, "too many distinct metric arrays");
+  ForeignMetric f;
+  if constexpr (sizeof(T) == 8) f.v64.assign(caller, caller + n);
+  else f.v32.assign(caller, caller + n);
+  X3D_HIP(hipMalloc(&f.dev, n * sizeof(T)));
+  X3D_HIP(hipMemcpy(f.dev, caller, n * sizeof(T), hipMemcpyHostToDevice));
+  g_foreign_metric.push_back(std::move(f));
+  *dev = (const T *)g_foreign_metric.back().dev;
   return X3D_OK;
 }
 
@@ -264,6 +294,9 @@ int x3d_finalize(void) {
     if (f.d32) (void)hipFree(f.d32);
   }
   g_foreign.clear();
+  for (ForeignMetric &f : g_foreign_metric)
+    if (f.dev) (void)hipFree(f.dev);
+  g_foreign_metric.clear();
   if (g_ctx->host_stage) (void)hipFree(g_ctx->host_stage);
   g_ctx.reset();
   return X3D_OK;

@@ -66,6 +66,31 @@ def test_dery(ctx, dtype, npaire):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_dery_caller_metric(ctx, dtype):
+    """dery multiplies by the ppy array its caller passes, as the reference
+    does, also when that array differs from x3d_init's (another libm, or a
+    different mapping): twice in a row, the second call from the cache."""
+    import torch
+    from incompact3d_amd import derive as D
+    name, p, S, X = ctx
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    u = seeded_field((p.nz, p.ny, p.nx), seed=31, lengths=(p.xlx, p.yly, p.zlz))
+    ppy = S.stret["ppy"]
+    mine = ppy * (1.0 + 1e-3 * np.cos(np.arange(p.ny)))
+    if dtype == "f32":
+        mine = mine.astype(np.float32).astype(np.float64)
+    ref = O.der1(S, u, "y", 1) / ppy[None, :, None] * mine[None, :, None]
+    c = S.d1["y"]
+    ud = torch.tensor(u, dtype=tdt, device="cuda")
+    fn = getattr(D, f"dery_{S.bc['y'][0]}{S.bc['y'][1]}")
+    for _ in range(2):
+        td = torch.empty_like(ud)
+        fn(td, ud, None, None, c["ffp"], c["fsp"], c["fwp"], mine, p.nx, p.ny, p.nz, 1)
+        err = rel_err(td.double().cpu().numpy(), ref)
+        assert err <= _tol(dtype, 2e-5), f"{name}: {err:.3e}"
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_staggered_y(ctx, dtype):
     import torch
     from incompact3d_amd import derive as D
