@@ -942,6 +942,8 @@ __global__ void __launch_bounds__(64) penta_y_kernel(cplx<T> *__restrict__ cw, c
   auto st = [&](int m, cplx<double> v) { at(m) = cplx<T>{(T)v.x, (T)v.y}; };
   // forward: window (e0, e1, e2) = rows m, m+1, m+2
   cplx<double> e0 = ld(0), e1 = ld(1), e2;
+  // unrolled so that the loads of later rows issue ahead of the chain
+#pragma unroll 4
   for (int m = 0; m + 2 < nh; ++m) {
     e2 = ld(m + 2);
     const double s1 = F1[(size_t)m * nxh], s2 = F2[(size_t)m * nxh];
@@ -959,6 +961,7 @@ __global__ void __launch_bounds__(64) penta_y_kernel(cplx<T> *__restrict__ cw, c
   st(n, xn);
   st(n - 1, xm);
   // back substitution: (xm, xn) = rows i+1, i+2
+#pragma unroll 4
   for (int i = nh - 3; i >= 0; --i) {
     const cplx<T> v = at(i);
     const double r = RP[(size_t)i * nxh], a = A1[(size_t)i * nxh], bb = B1[(size_t)i * nxh];
