@@ -187,7 +187,9 @@ X3D_DECL_STAG(float, _f32)
  * reference callers pass (ff*p for npaire=1, ...), enqueued on the library
  * stream without synchronising.  dir 0/1/2 = x/y/z; op 0 der1, 1 der2,
  * 2 dervp, 3 intervp, 4 derpv, 5 interpv; precision 0 = FP64, 1 = FP32;
- * (n1,n2,n3) are the extents of the input array. */
+ * (n1,n2,n3) are the extents of the input array.  On a stretched y mesh
+ * (istret != 0) the y operators multiply by the metric of x3d_init as the
+ * named entries do: der1 and derpv by ppy, dervp by ppyi. */
 int x3d_line_op_async(int dir, int op, int npaire, int precision, const void *u, void *t, int n1, int n2,
                       int n3);
 

@@ -525,12 +525,16 @@ int x3d_line_op_async(int dir, int op, int npaire, int precision, const void *u,
   if (dir < 0 || dir > 2 || op < 0 || op > 5 || (npaire != 0 && npaire != 1) || !u || !t || n1 <= 0 ||
       n2 <= 0 || n3 <= 0)
     return fail(X3D_ERR_ARG, "bad argument");
+  // on a stretched mesh the y operators that the named entries multiply by
+  // a metric do so here too: der1 by ppy, dervp by ppyi, derpv by ppy
+  const bool metric = dir == 1 && c->p.istret != 0 && (op == OP_DER1 || op == OP_DERVP || op == OP_DERPV);
   if (precision == 0) {
     LineCall<double> L;
     X3D_TRY(make_line_call<double>(*c, dir, op, npaire, &L));
     L.nx = n1; L.ny = n2; L.nz = n3;
     L.u = static_cast<const double *>(u);
     L.t = static_cast<double *>(t);
+    if (metric) X3D_TRY(y_metric<double>(*c, op == OP_DERVP, nullptr, L.n_out, &L.pp));
     return launch_line<double>(L, c->stream);
   }
   if (precision == 1) {
@@ -539,6 +543,7 @@ int x3d_line_op_async(int dir, int op, int npaire, int precision, const void *u,
     L.nx = n1; L.ny = n2; L.nz = n3;
     L.u = static_cast<const float *>(u);
     L.t = static_cast<float *>(t);
+    if (metric) X3D_TRY(y_metric<float>(*c, op == OP_DERVP, nullptr, L.n_out, &L.pp));
     return launch_line<float>(L, c->stream);
   }
   return fail(X3D_ERR_ARG, "precision must be 0 or 1");

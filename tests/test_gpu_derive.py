@@ -26,6 +26,16 @@ CONFIGS = [
     ("ywalls_long", dict(nx=16, ny=513, nz=8, ncly1=2, nclyn=2)),
     ("freeslip", dict(nx=17, ny=19, nz=21, nclx1=1, nclxn=1, ncly1=1, nclyn=1, nclz1=1, nclzn=1)),
     ("mixed", dict(nx=18, ny=22, nz=20, nclx1=1, nclxn=2, ncly1=2, nclyn=1, nclz1=2, nclzn=2)),
+    # the line lengths of the 256^3 step and of the 512^3 / 512x513x512
+    # microbenchmarks, with the launch shapes those grids use: x lines of
+    # 256 and 512 points (4 and 8 points per lane), y and z lines of 256 and
+    # 512 points over several blocks of lines, 513-point wall lines 64 wide
+    ("x256_lines", dict(nx=256, ny=16, nz=8)),
+    ("x512_lines", dict(nx=512, ny=8, nz=8)),
+    ("yz256_lines", dict(nx=64, ny=256, nz=256)),
+    ("y512_lines", dict(nx=64, ny=512, nz=8)),
+    ("z512_lines", dict(nx=64, ny=8, nz=512)),
+    ("ywalls_513x64", dict(nx=64, ny=513, nz=8, ncly1=2, nclyn=2)),
 ]
 
 

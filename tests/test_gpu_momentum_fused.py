@@ -21,7 +21,10 @@ from tests.helpers import rel_err
 
 pytestmark = pytest.mark.gpu
 
-GRIDS = [(64, 64, 64), (128, 32, 16), (64, 128, 32)]
+GRIDS = [(64, 64, 64), (128, 32, 16), (64, 128, 32),
+         # the line lengths of the 256^3 step and beyond: x sweeps with 4 and
+         # 8 points per lane, y and z sweeps of 32 and 64 segments
+         (256, 32, 16), (64, 256, 16), (64, 16, 256), (512, 16, 16), (64, 512, 16), (64, 16, 512)]
 
 
 def _solver(X, p, prec, unfused=False):

@@ -23,6 +23,20 @@ CASES = [
     ("p11x_bcz1_33x17x65", dict(nx=33, ny=17, nz=65, nclx1=1, nclxn=1, ncly1=1, nclyn=1, nclz1=1, nclzn=1)),
     ("p11x_bcz0_65x33x32", dict(nx=65, ny=33, nz=32, nclx1=1, nclxn=1, ncly1=1, nclyn=1)),
     ("p11x_bcz1_65", dict(nx=65, ny=65, nz=65, nclx1=1, nclxn=1, ncly1=1, nclyn=1, nclz1=1, nclzn=1)),
+    # the benchmark's grid and the 256- and 512-point passes of every
+    # direction (x: r2c/c2r of 128 and 256 complex points; y, z: the strided
+    # passes of 256 and 512 points, the z pass with the divide)
+    ("p000_256", dict(nx=256, ny=256, nz=256)),
+    ("p000_256x16x16", dict(nx=256, ny=16, nz=16)),
+    ("p000_16x256x16", dict(nx=16, ny=256, nz=16)),
+    ("p000_16x16x256", dict(nx=16, ny=16, nz=256)),
+    ("p000_512x16x16", dict(nx=512, ny=16, nz=16)),
+    ("p000_16x512x16", dict(nx=16, ny=512, nz=16)),
+    ("p000_16x16x512", dict(nx=16, ny=16, nz=512)),
+    # poisson_010 with the 512-point lines of the 512x513x512 walls grid
+    ("p010_16x513x16", dict(nx=16, ny=513, nz=16, ncly1=2, nclyn=2)),
+    ("p010_512x17x16", dict(nx=512, ny=17, nz=16, ncly1=2, nclyn=2)),
+    ("p010_16x17x512", dict(nx=16, ny=17, nz=512, ncly1=2, nclyn=2)),
 ]
 
 

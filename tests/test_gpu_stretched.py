@@ -20,6 +20,9 @@ CASES = [
     ("s1_freeslip_64x65x32", dict(nx=64, ny=65, nz=32, ncly1=1, nclyn=1, istret=1, beta=0.7)),
     ("s2_mixed_16x129x64", dict(nx=16, ny=129, nz=64, ncly1=2, nclyn=1, istret=2, beta=0.5)),
     ("s3_half_32x65x16", dict(nx=32, ny=65, nz=16, ncly1=2, nclyn=1, istret=3, beta=0.4)),
+    # the channel grid's x extent (256: the FP32 strided passes then give
+    # each block one of the nx/2+1-th lines) and its y extent
+    ("s2_walls_256x129x16", dict(nx=256, ny=129, nz=16, ncly1=2, nclyn=2, istret=2, beta=0.259)),
 ]
 
 
