@@ -33,8 +33,16 @@ Contents and pins (tests/test_oracle_les.py):
   Pinned by torch.optim.Adam on CPU.
 * ``closure_stress`` -- the learned stress with the Smagorinsky fallback where
   the predicted stress would feed energy to the resolved scales beyond the
-  bound: Pi = T_ij S_ij < -beta * 2 nut S:S.  Parity unpinned beyond the
-  per-point definition (no reference counterpart).
+  bound: Pi = T_ij S_ij < -beta * 2 nut S:S; optionally projected on the
+  strain (eddy-viscosity form, the solver's model 3).  No reference
+  counterpart.  Pinned as a composition: a network whose last layer is zero
+  returns its bias, so the stress, the fallback mask and the projection have
+  closed forms written with full 3x3 tensors (T:S = trace(T S)); the feature
+  order is pinned by velocity fields with a single nonzero gradient
+  component; the network itself is pinned against torch.
+* ``learned_sgs`` -- gradients, closure stress and the conservative
+  divergence composed; pinned through the pieces above and
+  ``stress_divergence`` (the conservative Smagorinsky pin).
 """
 from __future__ import annotations
 
@@ -303,9 +311,16 @@ def features(g, delta, gscale, delta0):
     return np.stack(cols, axis=1)
 
 
-def closure_stress(params, g, delta, gscale, delta0, cs, beta):
+def closure_stress(params, g, delta, gscale, delta0, cs, beta, project=False):
     """Learned T_ij (scaled back by delta^2 gscale^2) with the Smagorinsky
-    stress 2 nut S_ij where T_ij S_ij < -beta * 2 nut S:S."""
+    stress 2 nut S_ij where T_ij S_ij < -beta * 2 nut S:S.
+
+    project=False is the solver's model 2 (the network's full stress).
+    project=True is model 3: where the fallback does not apply, only the part
+    of T along the strain is kept, T = 2 nu_t S with the eddy viscosity
+    nu_t = max(T:S, 0) / (2 S:S) (0 where S:S = 0), so the closure dissipates
+    like an eddy-viscosity model of the conservative form
+    (les_models.f90:1327-1456) with a learned nu_t."""
     y = mlp_forward(params, features(g, delta, gscale, delta0))
     scale = delta * delta * gscale * gscale
     shape = g["xx"].shape
@@ -315,5 +330,21 @@ def closure_stress(params, g, delta, gscale, delta0, cs, beta):
     nut = (cs * delta) ** 2 * np.sqrt(2.0 * ss)
     pi = sum(t[k] * s[k] * (1.0 if k in ("xx", "yy", "zz") else 2.0) for k in COMPS)
     bad = pi < -beta * 2.0 * nut * ss
+    if project:
+        ok = (ss > 0.0) & (pi > 0.0)
+        nt = np.where(ok, pi / np.where(ok, 2.0 * ss, 1.0), 0.0)
+        t = {k: 2.0 * nt * s[k] for k in COMPS}
     out = {k: np.where(bad, 2.0 * nut * s[k], t[k]) for k in COMPS}
     return out, bad
+
+
+def learned_sgs(params, gscale, delta0, cs, beta, project=False):
+    """SGS force of the solver's learned closure (models 2 and 3): the nine
+    gradients of smag (les_models.f90:210-238), the closure stress above, and
+    its divergence with the parities of sgs_mom_conservative
+    (les_models.f90:1384-1440).  Returns fn(S, ux, uy, uz) for step_les."""
+    def fn(S, ux, uy, uz):
+        g = gradients(S, ux, uy, uz)
+        t, _ = closure_stress(params, g, filter_width(S.p), gscale, delta0, cs, beta, project)
+        return stress_divergence(S, t)
+    return fn

@@ -1381,10 +1381,11 @@ def div_stats(pp3):
 
 
 # ---------------------------------------------------------------------------
-# transeq.f90: momentum_rhs_eq (no LMN, no IBM, iimplicit=0, istret=0)
+# transeq.f90: momentum_rhs_eq (no LMN, no IBM, iimplicit=0)
 # ---------------------------------------------------------------------------
 def momentum_rhs(S: Schemes, ux, uy, uz):
-    """Skew-symmetric convection + nu*Laplacian (transeq.f90:73-470)."""
+    """Skew-symmetric convection + nu*Laplacian (transeq.f90:73-470); on a
+    stretched y mesh the y viscous term carries the metrics pp2y, pp4y."""
     xnu = S.p.xnu
     d1, d2 = der1, der2
     # X pencil (transeq.f90:114-146)
@@ -1431,6 +1432,14 @@ def momentum_rhs(S: Schemes, ux, uy, uz):
     td2 = d2(S, ux, "y", 1)
     te2 = d2(S, uy, "y", 0)
     tf2 = d2(S, uz, "y", 1)
+    if S.stret is not None:
+        # stretched y (transeq.f90:337-370): d2/dy2 = pp2y D2 - pp4y (ppy D1),
+        # the first derivative taken with the metric as dery applies it
+        pp2y = S.stret["pp2y"][:, None]
+        pp4y = S.stret["pp4y"][:, None]
+        td2 = td2 * pp2y - pp4y * d1(S, ux, "y", 1)
+        te2 = te2 * pp2y - pp4y * d1(S, uy, "y", 0)
+        tf2 = tf2 * pp2y - pp4y * d1(S, uz, "y", 1)
     ta2 = xnu * td2 + tg2
     tb2 = xnu * te2 + th2
     tc2 = xnu * tf2 + ti2
@@ -1614,3 +1623,150 @@ def step(S: Schemes, st: State):
     st.uy = st.uy - py
     st.uz = st.uz - pz
     return st
+
+
+# ---------------------------------------------------------------------------
+# Case-Channel.f90 and the wall (ncl=2) branches of navier.f90
+# ---------------------------------------------------------------------------
+MASK64 = (1 << 64) - 1
+
+
+def hash_uniform(seed, comp, idx):
+    """Uniform [0, 1) numbers of the channel initial noise: the SplitMix64
+    output function (Steele, Lea & Flood 2014) of the counter
+    seed * 2^40 + comp * 2^36 + idx (idx < 2^36), top 53 bits.  The
+    reference draws its noise with the compiler's random_number
+    (Case-Channel.f90:105-112), which no other program reproduces; this
+    counter-based generator is the same on the host and the device."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    z = (np.uint64((seed << 40) & MASK64) + np.uint64(comp << 36) + idx) * np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def channel_y(S: "Schemes"):
+    """y - yly/2 of the velocity-grid points (Case-Channel.f90:100-102):
+    yp(j) on a stretched mesh, (j-1) dy otherwise."""
+    p = S.p
+    if S.stret is not None:
+        return S.stret["yp"] - p.yly * 0.5
+    return np.arange(p.ny, dtype=np.float64) * p.dy - p.yly * 0.5
+
+
+def init_channel(S: "Schemes", init_noise, seed):
+    """Laminar Poiseuille profile plus modulated noise, iin=1 of
+    init_channel (Case-Channel.f90:96-128) with idir_stream=1:
+    u = init_noise * um * (2 r - 1) + 1 - y^2, v, w = init_noise * um *
+    (2 r - 1), um = exp(-0.2 y^2); r from hash_uniform of the point index
+    i + nx (j + ny k) and the component."""
+    p = S.p
+    y = channel_y(S)
+    um = np.exp(-0.2 * y * y)[None, :, None]
+    idx = np.arange(p.nx * p.ny * p.nz, dtype=np.uint64).reshape(p.nz, p.ny, p.nx)
+    r = [hash_uniform(seed, c, idx) for c in range(3)]
+    ux = init_noise * um * (2.0 * r[0] - 1.0) + (1.0 - y * y)[None, :, None]
+    uy = init_noise * um * (2.0 * r[1] - 1.0)
+    uz = init_noise * um * (2.0 * r[2] - 1.0)
+    return ux, uy, uz
+
+
+def channel_cfr(S: "Schemes", ux, constant):
+    """channel_cfr (Case-Channel.f90:220-261): the bulk velocity
+    ub = dy / (yly nx nz) sum u / ppy(j) (ppy = 1 on a uniform mesh,
+    variables.f90:1298) and u - can with can = -(constant - ub).  Returns
+    (new u, ub)."""
+    p = S.p
+    ppy = S.stret["ppy"] if S.stret is not None else np.ones(p.ny)
+    coeff = p.dy / (p.yly * float(p.nx * p.nz))
+    ub = float(np.sum(ux / ppy[None, :, None])) * coeff
+    can = -(constant - ub)
+    return ux - can, ub
+
+
+@dataclass
+class WallGrad:
+    """dpdxy1, dpdzy1, dpdxyn, dpdzyn of navier.f90 (the pressure gradient
+    on the y walls over gdt), shape (nz, nx)."""
+    x1: np.ndarray
+    z1: np.ndarray
+    xn: np.ndarray
+    zn: np.ndarray
+
+
+def wall_grad_zero(p: Params):
+    z = np.zeros((p.nz, p.nx))
+    return WallGrad(z.copy(), z.copy(), z.copy(), z.copy())
+
+
+def pre_correc_walls(p: Params, ux, uy, uz, wg: WallGrad, gdt):
+    """ncly=2 branches of pre_correc (navier.f90:616-686, no MHD) with the
+    no-slip wall velocity of the channel (byx1 = byy1 = byz1 = 0,
+    Case-Channel.f90:96): u(i,1,k) = dpdxy1 gdt, v = 0, w = dpdzy1 gdt,
+    and the same on j = ny.  Free-slip faces as pre_correc above."""
+    ux, uy, uz = pre_correc(p, ux, uy, uz)
+    if p.ncly1 == 2:
+        ux[:, 0, :] = 0.0 + wg.x1 * gdt
+        uy[:, 0, :] = 0.0
+        uz[:, 0, :] = 0.0 + wg.z1 * gdt
+    if p.nclyn == 2:
+        ux[:, -1, :] = 0.0 + wg.xn * gdt
+        uy[:, -1, :] = 0.0
+        uz[:, -1, :] = 0.0 + wg.zn * gdt
+    return ux, uy, uz
+
+
+def gradp_walls(p: Params, px, pz, wg: WallGrad, gdt):
+    """The wall planes gradp stores for the next pre_correc
+    (navier.f90:459-479): dpdxy1 = px(:,1,:)/gdt, dpdzy1 = pz(:,1,:)/gdt."""
+    if p.ncly1 == 2:
+        wg.x1 = px[:, 0, :] / gdt
+        wg.z1 = pz[:, 0, :] / gdt
+    if p.nclyn == 2:
+        wg.xn = px[:, -1, :] / gdt
+        wg.zn = pz[:, -1, :] / gdt
+    return wg
+
+
+def ab3_gdt(dt):
+    """gdt(1) of AB3: adt + bdt + cdt (variables.f90:1368-1372)."""
+    adt, bdt, cdt = ab3_coefficients(dt)
+    return adt + bdt + cdt
+
+
+def step_channel(S: Schemes, st: State, wg: WallGrad, cfr=2.0 / 3.0):
+    """One time step of the channel (xcompact3d.f90:46-87 with
+    itype_channel): per sub-step boundary_conditions (the constant flow
+    rate of channel_cfr when cfr is not None), the momentum right-hand side,
+    intt (AB3, or RK3 with st.hx = [f1, f2]), pre_correc with the wall
+    branches, divergence, Poisson, gradp storing the wall planes, cor_vel."""
+    p = S.p
+    st.itime += 1
+    dt = p.dt
+    if p.itimescheme == 5:
+        _, _, gdts = rk3_coefficients(dt)
+        subs = [(itr, gdts[itr - 1]) for itr in (1, 2, 3)]
+    else:
+        subs = [(1, ab3_gdt(dt))]
+    for itr, gdt in subs:
+        if cfr is not None:
+            st.ux, _ = channel_cfr(S, st.ux, cfr)
+        dux, duy, duz = momentum_rhs(S, st.ux, st.uy, st.uz)
+        st.hx[0], st.hy[0], st.hz[0] = dux, duy, duz
+        if p.itimescheme == 5:
+            st.ux = intt_rk3(st.ux, st.hx, itr, dt)
+            st.uy = intt_rk3(st.uy, st.hy, itr, dt)
+            st.uz = intt_rk3(st.uz, st.hz, itr, dt)
+        else:
+            st.ux = intt_ab3(st.ux, st.hx, st.itime, dt)
+            st.uy = intt_ab3(st.uy, st.hy, st.itime, dt)
+            st.uz = intt_ab3(st.uz, st.hz, st.itime, dt)
+        st.ux, st.uy, st.uz = pre_correc_walls(p, st.ux, st.uy, st.uz, wg, gdt)
+        pp3 = poisson(S, divergence(S, st.ux, st.uy, st.uz))
+        px, py, pz = gradp(S, pp3)
+        wg = gradp_walls(p, px, pz, wg, gdt)
+        st.ux = st.ux - px
+        st.uy = st.uy - py
+        st.uz = st.uz - pz
+    return st, wg

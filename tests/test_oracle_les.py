@@ -165,3 +165,90 @@ def test_closure_fallback_selects_smagorinsky_where_energy_injected():
     assert np.all(pi[bad] >= 0.0)           # replaced points dissipate
     assert np.all(pi[~bad] >= -1e-12)       # kept points never inject (beta = 0)
     assert np.allclose(t["xy"][bad], 2 * nut[bad] * s["xy"][bad])
+
+
+def _gradient_tensor(g):
+    """Full 3x3 velocity-gradient tensor G[i][j] = d u_i / d x_j from the
+    named gradients (name = component then direction)."""
+    ax = "xyz"
+    return [[g[ax[i] + ax[j]] for j in range(3)] for i in range(3)]
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.5])
+@pytest.mark.parametrize("project", [False, True])
+def test_closure_stress_composition_closed_form(beta, project):
+    """A network whose last layer is zero returns its bias c, so the closure
+    is T_ij = c_ij delta^2 gscale^2 whatever the hidden layers do.  The
+    stress, the fallback mask and the strain projection then have closed
+    forms, written here with full symmetric 3x3 tensors (T:S = sum_ij T_ij
+    S_ij, no factor-2 bookkeeping)."""
+    n = 8
+    rng = np.random.default_rng(21)
+    g = {k: rng.standard_normal((n, n, n)) for k in L.GRAD_ORDER}
+    params = L.init_params(5)
+    c = np.array([0.5, -0.25, 0.125, 0.375, -0.5, 0.0625], dtype=np.float32)   # BF16-exact
+    params[3] = (np.zeros_like(params[3][0]), c)
+    delta, gscale, delta0, cs = 0.1, 1.3, 0.07, 0.17
+    got, bad = L.closure_stress(params, g, delta, gscale, delta0, cs, beta, project)
+    scale = delta * delta * gscale * gscale
+    idx = {"xx": 0, "yy": 1, "zz": 2, "xy": 3, "xz": 4, "yz": 5}
+    cm = [[float(c[idx["".join(sorted("xyz"[i] + "xyz"[j]))]]) for j in range(3)] for i in range(3)]
+    G = _gradient_tensor(g)
+    Sm = [[0.5 * (G[i][j] + G[j][i]) for j in range(3)] for i in range(3)]
+    Tm = [[np.full((n, n, n), cm[i][j] * scale) for j in range(3)] for i in range(3)]
+    ts = sum(Tm[i][j] * Sm[i][j] for i in range(3) for j in range(3))
+    ss = sum(Sm[i][j] * Sm[i][j] for i in range(3) for j in range(3))
+    nut = (cs * delta) ** 2 * np.sqrt(2.0 * ss)
+    want_bad = ts < -beta * 2.0 * nut * ss
+    assert np.array_equal(bad, want_bad)
+    assert 0 < want_bad.sum() < want_bad.size
+    for name, (i, j) in L.PAIRS.items():
+        keep = 2.0 * np.maximum(ts, 0.0) / (2.0 * ss) * Sm[i][j] if project else Tm[i][j]
+        want = np.where(want_bad, 2.0 * nut * Sm[i][j], keep)
+        assert np.max(np.abs(got[name] - want)) <= 1e-13 * np.max(np.abs(want)), name
+    if project:
+        # the projection keeps the dissipation T:S of the kept points
+        tsg = sum(got[k] * L.strain(g)[k] * (1.0 if k in ("xx", "yy", "zz") else 2.0) for k in L.COMPS)
+        assert np.allclose(tsg[~want_bad], np.maximum(ts, 0.0)[~want_bad], rtol=1e-12, atol=1e-15)
+
+
+def test_closure_feature_order_single_gradient():
+    """u_i = sin(x_j) has the single nonzero gradient d u_i / d x_j, which
+    must land in feature 3*j + i (the documented device layout: g_xx g_yx
+    g_zx g_xy ...); the last feature is delta/delta0."""
+    n = 16
+    p = O.Params(n, n, n)
+    S = O.Schemes(p)
+    coords = np.meshgrid(np.arange(n) * p.dz, np.arange(n) * p.dy, np.arange(n) * p.dx, indexing="ij")
+    xyz = (coords[2], coords[1], coords[0])
+    for i in range(3):
+        for j in range(3):
+            u = [np.zeros((n, n, n)) for _ in range(3)]
+            u[i] = np.sin(xyz[j])
+            f = L.features(L.gradients(S, *u), 0.3, 2.0, 0.6)
+            nz = [k for k in range(9) if np.any(f[:, k] != 0.0)]
+            assert nz == [3 * j + i], (i, j, nz)
+            assert np.allclose(f[:, 3 * j + i], np.cos(xyz[j]).ravel() / 2.0, atol=1e-4)
+            assert np.all(f[:, 9] == 0.5)
+
+
+def test_learned_sgs_model3_is_eddy_viscosity_divergence():
+    """Model 3's force equals sgs_mom_conservative (les_models.f90:1327-1456)
+    with nu_t = max(T:S, 0)/(2 S:S) taken from the network's stress."""
+    n = 16
+    p = O.Params(n, n, n, re=1000.0)
+    S = O.Schemes(p)
+    from tests.helpers import perturbed_tgv
+    u = perturbed_tgv(p, seed=3, amp=0.2)
+    params = L.init_params(8)
+    gscale, delta0, cs, beta = 1.0, L.filter_width(p), 0.17, 1e30
+    force = L.learned_sgs(params, gscale, delta0, cs, beta, project=True)(S, *u)
+    g = L.gradients(S, *u)
+    t2, _ = L.closure_stress(params, g, L.filter_width(p), gscale, delta0, cs, beta, project=False)
+    s = L.strain(g)
+    ss = L.srt(s)
+    ts = sum(t2[k] * s[k] * (1.0 if k in ("xx", "yy", "zz") else 2.0) for k in L.COMPS)
+    nut = np.where((ss > 0) & (ts > 0), ts / (2.0 * np.where(ss > 0, ss, 1.0)), 0.0)
+    want = L.sgs_conservative(S, nut, s)
+    for a, b in zip(force, want):
+        assert np.max(np.abs(a - b)) <= 1e-12 * max(np.max(np.abs(b)), 1e-300)
