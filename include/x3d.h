@@ -73,8 +73,8 @@ int x3d_device_name(char *buf, int len);
  * caller can hand the same arrays back (the reference's module arrays).
  * name is the reference variable name: "ffx","fsx","fwx","ffxp",...,
  * "sfx",...,"cfx6","csx6","cwx6","cifxp6",... ("y"/"z" variants likewise),
- * and with istret != 0 the stretching.f90 arrays "yp","ypi","ppy","ppyi"
- * (stretching.f90:92-318).  Returns the array length in *n. */
+ * and with istret != 0 the stretching.f90 arrays "yp","ypi","ppy","ppyi",
+ * "pp2y","pp4y" (stretching.f90:92-318).  Returns the array length in *n. */
 int x3d_get_coefficients(const char *name, double *out, int cap, int *n);
 /* Scalar coefficient by reference name: "afix","bfix","alfaix","asix",...,
  * "acix6","alcaix6","aicix6","ailcaix6", "xnu", "dx", ... */
@@ -247,9 +247,15 @@ int x3d_host_poisson(double *rhs, const int *n1, const int *n2, const int *n3);
  * divergence / gradient + cor_vel as six; other grids (or the environment
  * variable X3D_UNFUSED_MOMENTUM set at create time) use one launch per
  * derivative.  Both paths compute the same quantities.
- * Boundaries: periodic or free-slip (ncl=1 at both ends) per direction, with
- * the Poisson solver of the grid (poisson_000, poisson_010, poisson_11x) and
- * pre_correc's zero normal velocity on free-slip faces (navier.f90:600-770).
+ * Boundaries: periodic or free-slip (ncl=1 at both ends) per direction, and
+ * in y also no-slip walls (ncl=2, either or both faces), with the Poisson
+ * solver of the grid (poisson_000, poisson_010 with its stretched branch,
+ * poisson_11x).  pre_correc sets the zero normal velocity on free-slip faces
+ * (navier.f90:600-770) and the wall velocity on no-slip faces from the
+ * pressure gradient gradp stored there (navier.f90:456-479, 616-690).
+ * A stretched y mesh (istret != 0) multiplies the y derivatives by ppy /
+ * ppyi and forms the y viscous term as pp2y deryy - pp4y dery
+ * (transeq.f90:337-370).
  * Time schemes: AB3 (itimescheme=3) and the low-storage RK3 (itimescheme=5,
  * time_integrators.f90:151-157, three projected sub-steps per step); the
  * fused sweeps run AB3 on periodic boxes. */
@@ -271,6 +277,23 @@ int x3d_solver_tgv_stats(x3d_solver *s, double out4[4]);
  * (navier.f90:346-361 with nlock=2 semantics, FP64). */
 int x3d_solver_divergence_stats(x3d_solver *s, double out2[2]);
 int x3d_solver_itime(x3d_solver *s, int *itime);
+/* Channel initial condition of init_channel (Case-Channel.f90:96-128, iin=1,
+ * idir_stream=1): u = init_noise um (2r-1) + 1 - y^2, v, w = init_noise um
+ * (2r-1), um = exp(-0.2 y^2), y the wall-normal coordinate (yp on a
+ * stretched mesh) minus yly/2.  The reference draws r with the compiler's
+ * random_number; here r is a counter-based SplitMix64 number of (seed, the
+ * component, the point index i + nx (j + ny k)), identical on every device
+ * and in the oracle.  seed < 2^24.  Resets itime and the AB3 history. */
+int x3d_solver_init_channel(x3d_solver *s, double init_noise, unsigned long long seed);
+/* Streamwise forcing of the channel, applied at every (sub-)step:
+ * mode 0 none; 1 constant flow rate, channel_cfr(ux, value) before the
+ * right-hand side (Case-Channel.f90:165-171, 220-261; value 2/3 in the
+ * reference); 2 constant pressure gradient, value added to the x right-hand
+ * side (cpg, fcpg of Case-Channel.f90:403-410). */
+int x3d_solver_set_forcing(x3d_solver *s, int mode, double value);
+/* Bulk velocity ub = dy / (yly nx nz) sum ux / ppy of channel_cfr
+ * (Case-Channel.f90:234-246), FP64. */
+int x3d_solver_bulk_velocity(x3d_solver *s, double *ub);
 /* Instrumentation, no counterpart in the reference: enqueue nrep times the
  * six fused y/z momentum sweeps of one step (z then y for each component,
  * on the current velocity, into the solver's work buffer; the velocity and
@@ -320,10 +343,12 @@ int x3d_mlp_train_step(x3d_mlp *m, const float *x, const float *t, int n, float 
 /* Gradient of the batch MSE into host_grad (x3d_mlp_nparams() floats), no update. */
 int x3d_mlp_gradient(x3d_mlp *m, const float *x, const float *t, int n, float *host_grad, float *loss);
 /* T_ij (device, precision 0 FP64 / 1 FP32) from the nine gradient fields g
- * (order as the features), with the Smagorinsky fallback; *nfallback (host,
- * may be NULL) counts the fallback points. */
+ * (order as the features), with the Smagorinsky fallback; project 0 keeps
+ * the network's stress (the solver's LES model 2), 1 its eddy-viscosity
+ * projection T = 2 nu_t S, nu_t = max(T:S, 0) / (2 S:S) (model 3);
+ * *nfallback (host, may be NULL) counts the fallback points. */
 int x3d_mlp_closure(x3d_mlp *m, int precision, const void *const *g, void *const *t, long long n, double gscale,
-                    double delta, double delta0, double cs, double beta, long long *nfallback);
+                    double delta, double delta0, double cs, double beta, int project, long long *nfallback);
 
 /* Top-hat filter of a device FP64 velocity (nx,ny,nz) over f^3 cells onto
  * (nx/f, ny/f, nz/f): ub[3] the filtered velocity, tau[6] (xx yy zz xy xz yz)

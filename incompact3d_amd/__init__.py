@@ -203,6 +203,19 @@ class Solver:
         check(_lib.lib().x3d_solver_momentum_yz(self._h, int(nrep), C.byref(b)), "momentum_yz")
         return b.value
 
+    def init_channel(self, init_noise: float = 0.125, seed: int = 20261202 % (1 << 24)):
+        """init_channel (Case-Channel.f90:96-128) with counter-based noise."""
+        check(_lib.lib().x3d_solver_init_channel(self._h, float(init_noise), int(seed)), "init_channel")
+
+    def set_forcing(self, mode: int, value: float = 2.0 / 3.0):
+        """0 none, 1 constant flow rate (channel_cfr), 2 constant pressure gradient."""
+        check(_lib.lib().x3d_solver_set_forcing(self._h, int(mode), float(value)), "set_forcing")
+
+    def bulk_velocity(self) -> float:
+        v = C.c_double()
+        check(_lib.lib().x3d_solver_bulk_velocity(self._h, C.byref(v)), "bulk_velocity")
+        return v.value
+
     @property
     def itime(self):
         v = C.c_int(0)
@@ -295,15 +308,16 @@ class Mlp:
                                           g.ctypes.data, C.byref(loss)), "mlp_gradient")
         return g, loss.value
 
-    def closure(self, g, gscale, delta, delta0, cs, beta):
-        """T_ij (6 tensors) from the nine gradient tensors g (FP64 or FP32)."""
+    def closure(self, g, gscale, delta, delta0, cs, beta, project: int = 0):
+        """T_ij (6 tensors) from the nine gradient tensors g (FP64 or FP32);
+        project=1 is the eddy-viscosity projection of LES model 3."""
         import torch
         t = [torch.empty_like(g[0]) for _ in range(6)]
         nfb = C.c_longlong(0)
         prec = 0 if g[0].dtype == torch.float64 else 1
         check(_lib.lib().x3d_mlp_closure(self._h, prec, _ptr_array(g), _ptr_array(t), int(g[0].numel()),
                                          float(gscale), float(delta), float(delta0), float(cs), float(beta),
-                                         C.byref(nfb)), "mlp_closure")
+                                         int(project), C.byref(nfb)), "mlp_closure")
         return t, nfb.value
 
 

@@ -92,6 +92,9 @@ def _declare(lib):
     lib.x3d_solver_divergence_stats.argtypes = [P, C.POINTER(C.c_double)]
     lib.x3d_solver_itime.argtypes = [P, IP]
     lib.x3d_solver_momentum_yz.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
+    lib.x3d_solver_init_channel.argtypes = [P, C.c_double, C.c_ulonglong]
+    lib.x3d_solver_set_forcing.argtypes = [P, C.c_int, C.c_double]
+    lib.x3d_solver_bulk_velocity.argtypes = [P, C.POINTER(C.c_double)]
     LLP = C.POINTER(C.c_longlong)
     lib.x3d_solver_set_les.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, P, C.c_double, C.c_double]
     lib.x3d_solver_sgs.argtypes = [P, P, P, P]
@@ -105,7 +108,7 @@ def _declare(lib):
     lib.x3d_mlp_train_step.argtypes = [P, P, P, C.c_int, C.c_float, C.POINTER(C.c_float)]
     lib.x3d_mlp_gradient.argtypes = [P, P, P, C.c_int, P, C.POINTER(C.c_float)]
     lib.x3d_mlp_closure.argtypes = [P, C.c_int, C.POINTER(P), C.POINTER(P), C.c_longlong, C.c_double, C.c_double,
-                                    C.c_double, C.c_double, C.c_double, LLP]
+                                    C.c_double, C.c_double, C.c_double, C.c_int, LLP]
     lib.x3d_box_filter.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(P), C.POINTER(P)]
 
 

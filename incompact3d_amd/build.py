@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libx3d.so")
 SOURCES = ["capi.cpp", "schemes.cpp", "lines_f64.hip", "lines_f32.hip", "fft.hip", "fields.hip", "solver.hip",
-           "momentum.hip", "projection.hip", "les.hip", "mlp.hip", "capi_les.cpp"]
+           "momentum.hip", "projection.hip", "les.hip", "mlp.hip", "capi_les.cpp", "channel.hip"]
 ARCH = os.environ.get("X3D_OFFLOAD_ARCH", "gfx950")
 
 # -packed-fp32-ops keeps the compiler from emitting packed FP32 arithmetic

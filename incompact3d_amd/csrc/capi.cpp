@@ -221,10 +221,12 @@ int stag_entry(int dir, int op, T *t, const T *u, const T *cf1, const T *cs1, co
 
 int find_array(const Context &c, const char *name, const std::vector<double> **out) {
   if (c.p.istret != 0) {
-    // stretching.f90 outputs (the reference's module arrays yp, ypi, ppy, ppyi)
+    // stretching.f90 outputs (the reference's module arrays yp, ypi, ppy,
+    // ppyi, pp2y, pp4y)
     const std::string s = name;
     const std::vector<double> *m = s == "yp" ? &c.yp : s == "ypi" ? &c.ypi : s == "ppy" ? &c.ppy
-                                 : s == "ppyi" ? &c.ppyi : nullptr;
+                                 : s == "ppyi" ? &c.ppyi : s == "pp2y" ? &c.pp2y : s == "pp4y" ? &c.pp4y
+                                 : nullptr;
     if (m) {
       *out = m;
       return X3D_OK;

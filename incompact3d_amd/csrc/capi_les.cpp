@@ -60,8 +60,9 @@ int x3d_mlp_gradient(x3d_mlp *m, const float *x, const float *t, int n, float *h
 }
 
 int x3d_mlp_closure(x3d_mlp *m, int precision, const void *const *g, void *const *t, long long n, double gscale,
-                    double delta, double delta0, double cs, double beta, long long *nfallback) {
+                    double delta, double delta0, double cs, double beta, int project, long long *nfallback) {
   if (!m || !g || !t || n <= 0) return fail(X3D_ERR_ARG, "bad argument");
+  if (project != 0 && project != 1) return fail(X3D_ERR_ARG, "project must be 0 or 1");
   for (int i = 0; i < 9; ++i)
     if (!g[i]) return fail(X3D_ERR_ARG, "null gradient field");
   for (int i = 0; i < 6; ++i)
@@ -80,14 +81,14 @@ int x3d_mlp_closure(x3d_mlp *m, int precision, const void *const *g, void *const
     for (int i = 0; i < 9; ++i) gg[i] = static_cast<const double *>(g[i]);
     for (int i = 0; i < 6; ++i) tt[i] = static_cast<double *>(t[i]);
     r = launch_closure<double>(reinterpret_cast<MlpDev *>(m), gg, tt, (size_t)n, gscale, delta / delta0, delta, cs,
-                               beta, 0, cnt, st);
+                               beta, project, cnt, st);
   } else if (precision == 1) {
     const float *gg[9];
     float *tt[6];
     for (int i = 0; i < 9; ++i) gg[i] = static_cast<const float *>(g[i]);
     for (int i = 0; i < 6; ++i) tt[i] = static_cast<float *>(t[i]);
     r = launch_closure<float>(reinterpret_cast<MlpDev *>(m), gg, tt, (size_t)n, gscale, delta / delta0, delta, cs,
-                              beta, 0, cnt, st);
+                              beta, project, cnt, st);
   } else {
     r = fail(X3D_ERR_ARG, "precision must be 0 or 1");
   }

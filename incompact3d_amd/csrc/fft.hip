@@ -1120,13 +1120,20 @@ static bool dispatch_x(bool fwd, int M, T *rhs, cplx<T> *cw, int nlines, int xdc
 }
 
 template <typename T>
-int poisson_solve(Context &c, T *rhs, hipStream_t st) {
-  auto *P = static_cast<PoissonPlan *>(c.poisson);
+size_t poisson_work_bytes(const Context &c) {
+  const auto *P = static_cast<const PoissonPlan *>(c.poisson);
+  return P ? (size_t)P->nxh * P->ny * P->nz * sizeof(cplx<T>) : 0;
+}
+template size_t poisson_work_bytes<double>(const Context &);
+template size_t poisson_work_bytes<float>(const Context &);
+
+// The five (stretched: seven) passes of the solve on the spectral work array
+// cw of poisson_work_bytes<T>(c) bytes; enqueues kernels only.
+template <typename T>
+int poisson_run(const Context &c, T *rhs, void *work, hipStream_t st) {
+  const auto *P = static_cast<const PoissonPlan *>(c.poisson);
   if (!P) return fail(X3D_ERR_STATE, "Poisson solver not initialised");
-  void **cwp = sizeof(T) == 8 ? &P->cw64 : &P->cw32;
-  const size_t ncw = (size_t)P->nxh * P->ny * P->nz;
-  if (!*cwp) X3D_HIP(hipMalloc(cwp, ncw * sizeof(cplx<T>)));
-  cplx<T> *cw = static_cast<cplx<T> *>(*cwp);
+  cplx<T> *cw = static_cast<cplx<T> *>(work);
   PoissonTables<T> tb;
   if constexpr (sizeof(T) == 8) tb = P->t64;
   else tb = P->t32;
@@ -1152,6 +1159,18 @@ int poisson_solve(Context &c, T *rhs, hipStream_t st) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("poisson launch: ") + hipGetErrorString(e));
   return X3D_OK;
+}
+template int poisson_run<double>(const Context &, double *, void *, hipStream_t);
+template int poisson_run<float>(const Context &, float *, void *, hipStream_t);
+
+// The library's own work array, allocated at the first solve of each precision.
+template <typename T>
+int poisson_solve(Context &c, T *rhs, hipStream_t st) {
+  auto *P = static_cast<PoissonPlan *>(c.poisson);
+  if (!P) return fail(X3D_ERR_STATE, "Poisson solver not initialised");
+  void **cwp = sizeof(T) == 8 ? &P->cw64 : &P->cw32;
+  if (!*cwp) X3D_HIP(hipMalloc(cwp, poisson_work_bytes<T>(c)));
+  return poisson_run<T>(c, rhs, *cwp, st);
 }
 
 template int poisson_solve<double>(Context &, double *, hipStream_t);

@@ -560,10 +560,16 @@ int compute_stretching(Context &c) {
   }
   c.ppy.assign(ny, 0.0);
   c.ppyi.assign(ny, 0.0);
+  c.pp2y.assign(ny, 0.0);
+  c.pp4y.assign(ny, 0.0);
   for (int j = 0; j < ny; ++j) {
     const double s1 = std::sin(pi * yeta[j]), s2 = std::sin(pi * yetai[j]);
     c.ppy[j] = yly * (alpha / pi + (1.0 / pi / beta) * s1 * s1);
     c.ppyi[j] = yly * (alpha / pi + (1.0 / pi / beta) * s2 * s2);
+    // stretching.f90:228-252: pp2y = ppy^2, pp4y = -2/beta cos sin (halved for istret=3)
+    c.pp2y[j] = c.ppy[j] * c.ppy[j];
+    c.pp4y[j] = (-2.0 / beta * std::cos(pi * yeta[j]) * s1);
+    if (ist == 3) c.pp4y[j] = c.pp4y[j] / 2.0;
   }
   return X3D_OK;
 }
@@ -583,6 +589,18 @@ int build_stretching(Context &c) {
   X3D_HIP(hipMemcpy(c.ppy32, f.data(), ny * sizeof(float), hipMemcpyHostToDevice));
   for (int j = 0; j < ny; ++j) f[j] = (float)c.ppyi[j];
   X3D_HIP(hipMemcpy(c.ppyi32, f.data(), ny * sizeof(float), hipMemcpyHostToDevice));
+  std::vector<double> pq(ny);
+  for (int j = 0; j < ny; ++j) pq[j] = c.pp4y[j] * c.ppy[j];
+  X3D_HIP(hipMalloc(&c.pp2y64, ny * sizeof(double)));
+  X3D_HIP(hipMalloc(&c.pp4yy64, ny * sizeof(double)));
+  X3D_HIP(hipMalloc(&c.pp2y32, ny * sizeof(float)));
+  X3D_HIP(hipMalloc(&c.pp4yy32, ny * sizeof(float)));
+  X3D_HIP(hipMemcpy(c.pp2y64, c.pp2y.data(), ny * sizeof(double), hipMemcpyHostToDevice));
+  X3D_HIP(hipMemcpy(c.pp4yy64, pq.data(), ny * sizeof(double), hipMemcpyHostToDevice));
+  for (int j = 0; j < ny; ++j) f[j] = (float)c.pp2y[j];
+  X3D_HIP(hipMemcpy(c.pp2y32, f.data(), ny * sizeof(float), hipMemcpyHostToDevice));
+  for (int j = 0; j < ny; ++j) f[j] = (float)pq[j];
+  X3D_HIP(hipMemcpy(c.pp4yy32, f.data(), ny * sizeof(float), hipMemcpyHostToDevice));
   return X3D_OK;
 }
 
@@ -591,8 +609,12 @@ void free_stretching(Context &c) {
   if (c.ppyi64) (void)hipFree(c.ppyi64);
   if (c.ppy32) (void)hipFree(c.ppy32);
   if (c.ppyi32) (void)hipFree(c.ppyi32);
-  c.ppy64 = c.ppyi64 = nullptr;
-  c.ppy32 = c.ppyi32 = nullptr;
+  if (c.pp2y64) (void)hipFree(c.pp2y64);
+  if (c.pp4yy64) (void)hipFree(c.pp4yy64);
+  if (c.pp2y32) (void)hipFree(c.pp2y32);
+  if (c.pp4yy32) (void)hipFree(c.pp4yy32);
+  c.ppy64 = c.ppyi64 = c.pp2y64 = c.pp4yy64 = nullptr;
+  c.ppy32 = c.ppyi32 = c.pp2y32 = c.pp4yy32 = nullptr;
 }
 
 }  // namespace x3d

@@ -1,7 +1,10 @@
-// Device-resident time step of the incompressible solver (periodic and free-slip boxes):
-//   calculate_transeq_rhs  transeq.f90:20-591   (skew-symmetric convection + nu lap)
+// Device-resident time step of the incompressible solver (periodic and
+// free-slip boxes, and no-slip walls in y as in the channel):
+//   boundary_conditions    Case-Channel.f90:153-261 (constant flow rate)
+//   calculate_transeq_rhs  transeq.f90:20-591   (skew-symmetric convection + nu lap,
+//                                                stretched-y metrics, forcing)
 //   int_time (AB3 / RK3)   time_integrators.f90:85-100, 151-157
-//   pre_correc             navier.f90:502-789   (free-slip faces)
+//   pre_correc             navier.f90:502-789   (free-slip faces, y walls)
 //   solve_poisson          navier.f90:27-141    (divergence, poisson, gradp)
 //   cor_vel                navier.f90:206-249
 // as in the time loop of xcompact3d.f90:29-102.  On grids the fused sweeps
@@ -13,7 +16,8 @@
 // precision 1 keeps an FP32 copy of the velocity for every derivative, FFT
 // and product, while the velocity itself is accumulated in FP64 (AB3 and
 // cor_vel) and the Poisson divide and reductions run in FP64.  RK3
-// (itimescheme=5) and free-slip boundaries run on the per-operator path.
+// (itimescheme=5), free-slip faces, y walls and stretched y meshes run on
+// the per-operator path.
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -39,6 +43,18 @@ int launch_tgv_sums(const T *ux, const T *uy, const T *uz, const T *const g[9], 
 int launch_zero_face(double *u, float *u32, int nx, int ny, int nz, int dir, int idx, hipStream_t st);
 template <typename T>
 int launch_max_mean(const T *a, size_t n, double *partial, double *out2, hipStream_t st);
+// channel.hip
+int launch_channel_init(double *ux, double *uy, double *uz, const double *yc, int nx, int ny, int nz, double noise,
+                        unsigned long long seed, hipStream_t st);
+int launch_bulk(const double *u, const double *ppy, int nx, int ny, int nz, double coeff, double constant,
+                double *partial, double *out2, hipStream_t st);
+int launch_cfr_shift(double *u, float *u32, const double *out2, size_t n, hipStream_t st);
+template <typename T>
+int launch_add_const(T *f, double c, size_t n, hipStream_t st);
+int launch_wall_velocity(double *const u[3], float *const v[3], const double *dpdx, const double *dpdz, double gdt,
+                         int nx, int ny, int nz, int j, hipStream_t st);
+template <typename T>
+int launch_wall_store(const T *g, double *dst, double gdt, int nx, int ny, int nz, int j, hipStream_t st);
 
 }  // namespace x3d
 
@@ -66,16 +82,49 @@ struct x3d_solver {
   x3d::MlpDev *mlp = nullptr;  // not owned
   void *lesw = nullptr;        // 20 fields: g[9], s or T [6], nut, ta, sgs[3]
   int *nfb = nullptr;          // fallback counter of the learned closure (device)
+  // no-slip walls in y (ncly1/nclyn = 2): the pressure gradient on each wall
+  // over gdt that gradp stores for the next pre_correc, dpdxy1, dpdzy1,
+  // dpdxyn, dpdzyn of navier.f90:456-479, nz*nx each, [k*nx + i]
+  bool wall1 = false, walln = false;
+  double *wallg = nullptr;
+  // streamwise forcing of the channel: 0 none, 1 constant flow rate
+  // (channel_cfr towards fvalue, Case-Channel.f90:165-171), 2 constant
+  // pressure gradient fvalue added to the x right-hand side (cpg, :403-410)
+  int forcing = 0;
+  double fvalue = 0.0;
+  double *ycoord = nullptr;    // wall-normal coordinate of init_channel
 };
 
 namespace x3d {
 namespace {
 
+// Device copies of the stretched-y metrics (nullptr on a uniform mesh).
+enum { M_PPY = 0, M_PPYI, M_PP2Y, M_PP4YY };
+template <typename T>
+const T *metric(const Context &c, int which) {
+  if (c.p.istret == 0) return nullptr;
+  if constexpr (sizeof(T) == 8) {
+    const double *m[4] = {c.ppy64, c.ppyi64, c.pp2y64, c.pp4yy64};
+    return (const T *)m[which];
+  } else {
+    const float *m[4] = {c.ppy32, c.ppyi32, c.pp2y32, c.pp4yy32};
+    return (const T *)m[which];
+  }
+}
+
+// One line operator with the epilogue t = ac*acc + rc*(mul*result).  On a
+// stretched y mesh the y operators multiply by the metric the reference
+// passes them: dery and derypv by ppy, deryvp by ppyi (derive.f90:409-417,
+// 4572-4578, 4905-4911); deryy and the interpolations by none.  pp, when
+// given, replaces that metric.
 template <typename T>
 int line(const Context &c, int dir, int op, int npaire, const T *u, const T *u2, T *t, const T *acc, const T *mul,
-         double ac, double rc, int nx, int ny, int nz) {
+         double ac, double rc, int nx, int ny, int nz, const T *pp = nullptr) {
   LineCall<T> L;
   X3D_TRY(make_line_call<T>(c, dir, op, npaire, &L));
+  if (pp) L.pp = pp;
+  else if (dir == 1 && (op == OP_DER1 || op == OP_DERPV)) L.pp = metric<T>(c, M_PPY);
+  else if (dir == 1 && op == OP_DERVP) L.pp = metric<T>(c, M_PPYI);
   L.nx = nx;
   L.ny = ny;
   L.nz = nz;
@@ -111,7 +160,13 @@ int momentum_component(const Context &c, const T *const u[3], int comp, T *W, T 
       X3D_TRY(line<T>(c, d, OP_DER1, pp, u[d], uc, W, W, nullptr, 1.0, -0.5, nx, ny, nz));
       X3D_TRY(line<T>(c, d, OP_DER1, pv, uc, nullptr, W, W, u[d], 1.0, -0.5, nx, ny, nz));
       T *dst = d == 0 ? out : W;
-      X3D_TRY(line<T>(c, d, OP_DER2, pv, uc, nullptr, dst, W, nullptr, 1.0, xnu, nx, ny, nz));
+      if (d == 1 && c.p.istret != 0) {
+        // transeq.f90:337-370: nu (pp2y deryy u - pp4y dery u), dery with ppy
+        X3D_TRY(line<T>(c, d, OP_DER2, pv, uc, nullptr, W, W, nullptr, 1.0, xnu, nx, ny, nz, metric<T>(c, M_PP2Y)));
+        X3D_TRY(line<T>(c, d, OP_DER1, pv, uc, nullptr, W, W, nullptr, 1.0, -xnu, nx, ny, nz, metric<T>(c, M_PP4YY)));
+      } else {
+        X3D_TRY(line<T>(c, d, OP_DER2, pv, uc, nullptr, dst, W, nullptr, 1.0, xnu, nx, ny, nz));
+      }
     }
   }
   return X3D_OK;
@@ -195,10 +250,12 @@ int divergence(const Context &c, const T *const u[3], T *const W[4], int nx, int
 }
 
 // gradp (navier.f90:386-431) with cor_vel fused: u_i <- u_i - grad_i p.
-// pp3 (pressure grid) in W[0]; for FP32 compute the gradient goes to W[2]
-// and is subtracted from the FP64 velocity.
+// pp3 (pressure grid) in W[0]; for FP32 compute, or with walls in y, the
+// gradient goes to W[2] and is subtracted from the FP64 velocity; on the
+// walls its x and z components over gdt are kept for pre_correc
+// (navier.f90:456-479).
 template <typename T>
-int gradp_correct(const Context &c, x3d_solver *S, T *const u[3], T *const W[4]) {
+int gradp_correct(const Context &c, x3d_solver *S, T *const u[3], T *const W[4], double gdt) {
   const int nx = S->nx, ny = S->ny, nz = S->nz, nxm = c.nxm, nym = c.nym, nzm = c.nzm;
   X3D_TRY(line<T>(c, 2, OP_INTERPV, 1, W[0], nullptr, W[1], nullptr, nullptr, 0, 1, nxm, nym, nzm));
   X3D_TRY(line<T>(c, 2, OP_DERPV, 1, W[0], nullptr, W[2], nullptr, nullptr, 0, 1, nxm, nym, nzm));
@@ -208,13 +265,21 @@ int gradp_correct(const Context &c, x3d_solver *S, T *const u[3], T *const W[4])
   // px from W[3], py from W[0], pz from W[1]
   const T *src[3] = {W[3], W[0], W[1]};
   const int ops[3] = {OP_DERPV, OP_INTERPV, OP_INTERPV};
+  const bool walls = S->wall1 || S->walln;
+  const size_t np = (size_t)nx * nz;
   for (int i = 0; i < 3; ++i) {
-    if constexpr (sizeof(T) == 8) {
+    if (sizeof(T) == 8 && !walls) {
       X3D_TRY(line<T>(c, 0, ops[i], 1, src[i], nullptr, u[i], u[i], nullptr, 1.0, -1.0, nxm, ny, nz));
-    } else {
-      X3D_TRY(line<T>(c, 0, ops[i], 1, src[i], nullptr, W[2], nullptr, nullptr, 0, 1, nxm, ny, nz));
-      X3D_TRY(launch_corvel<float>(S->u64[i], (const float *)W[2], S->u32[i], S->n, c.stream));
+      continue;
     }
+    X3D_TRY(line<T>(c, 0, ops[i], 1, src[i], nullptr, W[2], nullptr, nullptr, 0, 1, nxm, ny, nz));
+    if (walls && i != 1) {
+      // dpdxy1 / dpdzy1 (j = 1) and dpdxyn / dpdzyn (j = ny)
+      const int q = i == 0 ? 0 : 1;
+      if (S->wall1) X3D_TRY(launch_wall_store<T>(W[2], S->wallg + q * np, gdt, nx, ny, nz, 0, c.stream));
+      if (S->walln) X3D_TRY(launch_wall_store<T>(W[2], S->wallg + (2 + q) * np, gdt, nx, ny, nz, ny - 1, c.stream));
+    }
+    X3D_TRY(launch_corvel<T>(S->u64[i], W[2], sizeof(T) == 8 ? nullptr : S->u32[i], S->n, c.stream));
   }
   return X3D_OK;
 }
@@ -245,8 +310,11 @@ int momentum_yz(Context &c, x3d_solver *S, int nrep, double *bytes) {
 }
 
 // pre_correc (navier.f90:502-789) for the boundaries this solver covers:
-// zero normal velocity on free-slip faces (nothing for periodic ones)
-int pre_correc(const Context &c, x3d_solver *S) {
+// zero normal velocity on free-slip faces (nothing for periodic ones); on a
+// no-slip y wall (navier.f90:616-690) u = dpdxy gdt, v = 0, w = dpdzy gdt,
+// the wall velocity that the pressure correction of this sub-step brings
+// back to zero.
+int pre_correc(const Context &c, x3d_solver *S, double gdt) {
   float *const u32[3] = {S->u32[0], S->u32[1], S->u32[2]};
   const int nn[3] = {S->nx, S->ny, S->nz};
   for (int d = 0; d < 3; ++d) {
@@ -254,7 +322,32 @@ int pre_correc(const Context &c, x3d_solver *S) {
     if (c.dir[d].bcn == 1)
       X3D_TRY(launch_zero_face(S->u64[d], u32[d], S->nx, S->ny, S->nz, d, nn[d] - 1, c.stream));
   }
+  const size_t np = (size_t)S->nx * S->nz;
+  double *const u64[3] = {S->u64[0], S->u64[1], S->u64[2]};
+  if (S->wall1)
+    X3D_TRY(launch_wall_velocity(u64, u32, S->wallg, S->wallg + np, gdt, S->nx, S->ny, S->nz, 0, c.stream));
+  if (S->walln)
+    X3D_TRY(launch_wall_velocity(u64, u32, S->wallg + 2 * np, S->wallg + 3 * np, gdt, S->nx, S->ny, S->nz, S->ny - 1,
+                                 c.stream));
   return X3D_OK;
+}
+
+// boundary_conditions of the channel (Case-Channel.f90:165-171): the
+// constant flow rate of channel_cfr, u <- u + (constant - ub) with
+// ub = dy / (yly nx nz) sum u / ppy (Case-Channel.f90:220-261)
+int channel_cfr(const Context &c, x3d_solver *S) {
+  if (S->forcing != 1) return X3D_OK;
+  const double coeff = c.dir[1].d / (c.p.yly * (double)((long long)S->nx * S->nz));
+  X3D_TRY(launch_bulk(S->u64[0], c.p.istret != 0 ? c.ppy64 : nullptr, S->nx, S->ny, S->nz, coeff, S->fvalue,
+                      S->partial, S->red, c.stream));
+  return launch_cfr_shift(S->u64[0], S->u32[0], S->red, S->n, c.stream);
+}
+
+// momentum_forcing_channel with cpg (Case-Channel.f90:403-410): dux1 += fcpg
+template <typename T>
+int channel_cpg(const Context &c, x3d_solver *S, T *f1x) {
+  if (S->forcing != 2) return X3D_OK;
+  return launch_add_const<T>(f1x, S->fvalue, S->n, c.stream);
 }
 
 // One RK3 step (itimescheme=5, xcompact3d.f90:46-87): three sub-steps of
@@ -272,8 +365,10 @@ int step_rk3(Context &c, x3d_solver *S) {
   const double dt = c.p.dt;
   const double adt[3] = {(8.0 / 15.0) * dt, (5.0 / 12.0) * dt, (3.0 / 4.0) * dt};
   const double bdt[3] = {0.0, (-17.0 / 60.0) * dt, (-5.0 / 12.0) * dt};
+  const double gdt[3] = {adt[0], adt[1] + bdt[1], adt[2] + bdt[2]};
   const T *cu[3] = {u[0], u[1], u[2]};
   for (int itr = 0; itr < 3; ++itr) {
+    X3D_TRY(channel_cfr(c, S));
     T *sgs[3] = {nullptr, nullptr, nullptr};
     if (S->les) {
       for (int i = 0; i < 3; ++i) sgs[i] = (T *)S->lesw + (17 + i) * S->n;
@@ -283,6 +378,7 @@ int step_rk3(Context &c, x3d_solver *S) {
       T *f1 = (T *)S->hist[comp][S->slot[0]];
       X3D_TRY(momentum_component<T>(c, cu, comp, W[0], f1, nx, ny, nz));
       if (S->les) X3D_TRY(launch_add<T>(f1, sgs[comp], S->n, c.stream));
+      if (comp == 0) X3D_TRY(channel_cpg<T>(c, S, f1));
     }
     for (int comp = 0; comp < 3; ++comp) {
       const T *f1 = (T *)S->hist[comp][S->slot[0]];
@@ -292,10 +388,10 @@ int step_rk3(Context &c, x3d_solver *S) {
                            sizeof(T) == 8 ? nullptr : S->u32[comp], S->n, c.stream));
     }
     std::swap(S->slot[0], S->slot[1]);   // f2 <- f1
-    X3D_TRY(pre_correc(c, S));
+    X3D_TRY(pre_correc(c, S, gdt[itr]));
     X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));
     X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
-    X3D_TRY(gradp_correct<T>(c, S, u, W));
+    X3D_TRY(gradp_correct<T>(c, S, u, W, gdt[itr]));
   }
   return X3D_OK;
 }
@@ -318,6 +414,9 @@ int step_once(Context &c, x3d_solver *S) {
   } else {
     a = (23.0 / 12.0) * dt; b = -(16.0 / 12.0) * dt; cc = (5.0 / 12.0) * dt; nterms = 3;
   }
+  // gdt(1) of variables.f90:1368-1372, the same at every step
+  const double gdt = (23.0 / 12.0) * dt + (-(16.0 / 12.0) * dt) + (5.0 / 12.0) * dt;
+  X3D_TRY(channel_cfr(c, S));
   // transeq (z, y, x sweeps per component) with AB3 fused into the x sweep;
   // the new velocity goes to the second buffer set
   const T *cu[3] = {u[0], u[1], u[2]};
@@ -356,6 +455,7 @@ int step_once(Context &c, x3d_solver *S) {
       T *f1 = (T *)S->hist[comp][S->slot[0]];
       X3D_TRY(momentum_component<T>(c, cu, comp, W[0], f1, nx, ny, nz));
       if (S->les) X3D_TRY(launch_add<T>(f1, sgs[comp], S->n, c.stream));
+      if (comp == 0) X3D_TRY(channel_cpg<T>(c, S, f1));
     }
     for (int comp = 0; comp < 3; ++comp) {
       const T *f1 = (T *)S->hist[comp][S->slot[0]];
@@ -396,10 +496,10 @@ int step_once(Context &c, x3d_solver *S) {
     X3D_TRY(launch_projection<T>(c, SW_GRAD_X, gx_in, onone, u64, sizeof(T) == 8 ? nullptr : u32,
                                  sizeof(T) == 8 ? nullptr : du, nx, ny, nz, c.stream));
   } else {
-    X3D_TRY(pre_correc(c, S));
+    X3D_TRY(pre_correc(c, S, gdt));
     X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));
     X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
-    X3D_TRY(gradp_correct<T>(c, S, u, W));
+    X3D_TRY(gradp_correct<T>(c, S, u, W, gdt));
   }
   return X3D_OK;
 }
@@ -428,8 +528,6 @@ int tgv_stats(Context &c, x3d_solver *S, double out4[4]) {
     }
   const T *gg[9];
   for (int i = 0; i < 9; ++i) gg[i] = g[i];
-  if (!S->partial) X3D_HIP(hipMalloc(&S->partial, 8 * 1024 * sizeof(double)));
-  if (!S->red) X3D_HIP(hipMalloc(&S->red, 8 * sizeof(double)));
   // free-slip directions count nm = n-1 cells (Case-TGV.f90:213-242)
   const int nxc = c.dir[0].bc1 == 1 ? c.nxm : nx, nyc = c.dir[1].bc1 == 1 ? c.nym : ny,
             nzc = c.dir[2].bc1 == 1 ? c.nzm : nz;
@@ -464,6 +562,9 @@ int alloc_solver(Context &c, x3d_solver *S) {
   const size_t np = (size_t)c.nxm * c.nym * c.nzm;
   const size_t nw = n > np ? n : np;
   for (int i = 0; i < 4; ++i) X3D_HIP(hipMalloc(&S->w[i], nw * es));
+  X3D_HIP(hipMalloc(&S->partial, 8 * 1024 * sizeof(double)));
+  X3D_HIP(hipMalloc(&S->red, 8 * sizeof(double)));
+  if (S->wall1 || S->walln) X3D_HIP(hipMalloc(&S->wallg, 4 * (size_t)S->nx * S->nz * sizeof(double)));
   X3D_HIP(hipStreamSynchronize(c.stream));
   return X3D_OK;
 }
@@ -486,6 +587,8 @@ void free_solver(x3d_solver *S) {
   if (S->nfb) (void)hipFree(S->nfb);
   if (S->partial) (void)hipFree(S->partial);
   if (S->red) (void)hipFree(S->red);
+  if (S->wallg) (void)hipFree(S->wallg);
+  if (S->ycoord) (void)hipFree(S->ycoord);
 }
 
 int reset_history(Context &c, x3d_solver *S) {
@@ -498,6 +601,8 @@ int reset_history(Context &c, x3d_solver *S) {
   S->slot[1] = 1;
   S->slot[2] = 2;
   S->itime = 0;
+  // dpdxy1 .. dpdzyn start at zero (the wall velocity of the first pre_correc)
+  if (S->wallg) X3D_HIP(hipMemsetAsync(S->wallg, 0, 4 * (size_t)S->nx * S->nz * sizeof(double), c.stream));
   if (S->precision == 1)
     for (int i = 0; i < 3; ++i) X3D_TRY(launch_cast(S->u64[i], S->u32[i], S->n, c.stream));
   X3D_HIP(hipStreamSynchronize(c.stream));
@@ -518,12 +623,13 @@ int x3d_solver_create(int precision, x3d_solver **out) {
   if (precision != 0 && precision != 1) return fail(X3D_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 bulk)");
   for (int d = 0; d < 3; ++d) {
     const int b1 = c->dir[d].bc1, bn = c->dir[d].bcn;
+    if (d == 1 && b1 != 0 && bn != 0) continue;   // free-slip or no-slip faces in y, in any pairing
     if (!((b1 == 0 && bn == 0) || (b1 == 1 && bn == 1)))
-      return fail(X3D_ERR_UNSUPPORTED, "the device time step covers periodic and free-slip (ncl=1) boundaries");
+      return fail(X3D_ERR_UNSUPPORTED,
+                  "the device time step covers periodic and free-slip (ncl=1) boundaries, and no-slip walls "
+                  "(ncl=2) in y only");
   }
   if (!c->poisson) return fail(X3D_ERR_UNSUPPORTED, "no Poisson solver for this grid");
-  if (c->p.istret != 0)
-    return fail(X3D_ERR_UNSUPPORTED, "the device time step has no stretched-y momentum terms (istret != 0)");
   if (c->p.itimescheme != 3 && c->p.itimescheme != 5)
     return fail(X3D_ERR_UNSUPPORTED, "the device time step implements AB3 (itimescheme=3) and RK3 (itimescheme=5)");
   auto *S = new x3d_solver();
@@ -532,7 +638,9 @@ int x3d_solver_create(int precision, x3d_solver **out) {
   S->ny = c->p.ny;
   S->nz = c->p.nz;
   S->n = (size_t)S->nx * S->ny * S->nz;
-  S->fused = c->p.itimescheme == 3 && momentum_fusable(*c) && projection_fusable(*c) &&
+  S->wall1 = c->dir[1].bc1 == 2;
+  S->walln = c->dir[1].bcn == 2;
+  S->fused = c->p.itimescheme == 3 && c->p.istret == 0 && momentum_fusable(*c) && projection_fusable(*c) &&
              getenv("X3D_UNFUSED_MOMENTUM") == nullptr;
   int r = alloc_solver(*c, S);
   if (r != X3D_OK) {
@@ -619,8 +727,6 @@ int x3d_solver_divergence_stats(x3d_solver *S, double out2[2]) {
   double *W[4] = {S->w64[0], S->w64[1], S->w64[2], S->w64[3]};
   const double *u[3] = {S->u64[0], S->u64[1], S->u64[2]};
   X3D_TRY(divergence<double>(*c, u, W, S->nx, S->ny, S->nz));
-  if (!S->partial) X3D_HIP(hipMalloc(&S->partial, 8 * 1024 * sizeof(double)));
-  if (!S->red) X3D_HIP(hipMalloc(&S->red, 8 * sizeof(double)));
   X3D_TRY(launch_max_mean<double>(W[0], np, S->partial, S->red, c->stream));
   X3D_HIP(hipMemcpyAsync(out2, S->red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   X3D_HIP(hipStreamSynchronize(c->stream));
@@ -638,6 +744,8 @@ int x3d_solver_set_les(x3d_solver *S, int model, double cs, int iconserv, double
   if (model >= 2 && (!mlp || !(gscale > 0) || !(delta0 > 0)))
     return fail(X3D_ERR_ARG, "learned closure needs a network and positive gscale, delta0");
   if (!(cs >= 0)) return fail(X3D_ERR_ARG, "cs must be >= 0");
+  if (model && c->p.istret != 0)
+    return fail(X3D_ERR_UNSUPPORTED, "LES on a stretched y mesh (the filter width varies with y) is not provided");
   if (model && !S->lesw) {
     const size_t es = S->precision == 0 ? 8 : 4;
     X3D_HIP(hipMalloc(&S->lesw, 20 * S->n * es));
@@ -696,6 +804,47 @@ int x3d_solver_momentum_yz(x3d_solver *S, int nrep, double *bytes_per_launch) {
   if (!S->fused) return fail(X3D_ERR_UNSUPPORTED, "momentum sweeps: the solver does not use the fused path");
   if (S->precision == 0) return momentum_yz<double>(*c, S, nrep, bytes_per_launch);
   return momentum_yz<float>(*c, S, nrep, bytes_per_launch);
+}
+
+int x3d_solver_init_channel(x3d_solver *S, double init_noise, unsigned long long seed) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S) return fail(X3D_ERR_ARG, "null solver");
+  if (!(init_noise >= 0)) return fail(X3D_ERR_ARG, "init_noise must be >= 0");
+  if (seed >= (1ull << 24)) return fail(X3D_ERR_ARG, "seed must be below 2^24");
+  // y - yly/2 of the velocity points (Case-Channel.f90:100-102)
+  std::vector<double> yc(S->ny);
+  for (int j = 0; j < S->ny; ++j)
+    yc[j] = c->p.istret != 0 ? c->yp[j] - c->p.yly * 0.5 : (double)j * c->dir[1].d - c->p.yly * 0.5;
+  if (!S->ycoord) X3D_HIP(hipMalloc(&S->ycoord, S->ny * sizeof(double)));
+  X3D_HIP(hipMemcpyAsync(S->ycoord, yc.data(), S->ny * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  X3D_TRY(launch_channel_init(S->u64[0], S->u64[1], S->u64[2], S->ycoord, S->nx, S->ny, S->nz, init_noise, seed,
+                              c->stream));
+  return reset_history(*c, S);
+}
+
+int x3d_solver_set_forcing(x3d_solver *S, int mode, double value) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S) return fail(X3D_ERR_ARG, "null solver");
+  if (mode < 0 || mode > 2) return fail(X3D_ERR_ARG, "forcing must be 0 (none), 1 (flow rate) or 2 (pressure gradient)");
+  if (mode == 1 && (c->dir[1].bc1 == 0 || c->dir[1].bcn == 0))
+    return fail(X3D_ERR_UNSUPPORTED, "the constant flow rate is defined for a y-bounded channel");
+  S->forcing = mode;
+  S->fvalue = value;
+  return X3D_OK;
+}
+
+int x3d_solver_bulk_velocity(x3d_solver *S, double *ub) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || !ub) return fail(X3D_ERR_ARG, "null pointer");
+  const double coeff = c->dir[1].d / (c->p.yly * (double)((long long)S->nx * S->nz));
+  X3D_TRY(launch_bulk(S->u64[0], c->p.istret != 0 ? c->ppy64 : nullptr, S->nx, S->ny, S->nz, coeff, 0.0, S->partial,
+                      S->red, c->stream));
+  X3D_HIP(hipMemcpyAsync(ub, S->red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  X3D_HIP(hipStreamSynchronize(c->stream));
+  return X3D_OK;
 }
 
 int x3d_solver_itime(x3d_solver *S, int *itime) {

@@ -107,9 +107,13 @@ struct Context {
   // stretched y mesh (istret != 0): the metrics of stretching.f90 on the
   // velocity (ppy) and pressure (ppyi) grids, host FP64 and device copies
   double alpha = 0;
-  std::vector<double> yp, ypi, ppy, ppyi;
+  std::vector<double> yp, ypi, ppy, ppyi, pp2y, pp4y;
   double *ppy64 = nullptr, *ppyi64 = nullptr;
   float *ppy32 = nullptr, *ppyi32 = nullptr;
+  // the y viscous term of transeq.f90:337-370 on a stretched mesh,
+  // pp2y D2 - pp4y (ppy D1): device copies of pp2y and of pp4y*ppy
+  double *pp2y64 = nullptr, *pp4yy64 = nullptr;
+  float *pp2y32 = nullptr, *pp4yy32 = nullptr;
   // Poisson state (fft.hip)
   void *poisson = nullptr;
   // scratch for host-memory entries
@@ -172,6 +176,11 @@ int poisson_setup(Context &c);
 void poisson_free(Context &c);
 template <typename T>
 int poisson_solve(Context &c, T *rhs, hipStream_t st);
+// the solve on a caller-provided spectral work array of poisson_work_bytes
+template <typename T>
+size_t poisson_work_bytes(const Context &c);
+template <typename T>
+int poisson_run(const Context &c, T *rhs, void *work, hipStream_t st);
 
 // ---------------------------------------------------------------- fused sweeps
 // momentum.hip: the transeq right-hand side of component comp along dir

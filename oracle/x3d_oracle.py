@@ -1735,12 +1735,14 @@ def ab3_gdt(dt):
     return adt + bdt + cdt
 
 
-def step_channel(S: Schemes, st: State, wg: WallGrad, cfr=2.0 / 3.0):
+def step_channel(S: Schemes, st: State, wg: WallGrad, cfr=2.0 / 3.0, fcpg=None):
     """One time step of the channel (xcompact3d.f90:46-87 with
     itype_channel): per sub-step boundary_conditions (the constant flow
-    rate of channel_cfr when cfr is not None), the momentum right-hand side,
-    intt (AB3, or RK3 with st.hx = [f1, f2]), pre_correc with the wall
-    branches, divergence, Poisson, gradp storing the wall planes, cor_vel."""
+    rate of channel_cfr when cfr is not None), the momentum right-hand side
+    (plus the constant pressure gradient fcpg on dux when fcpg is not None,
+    momentum_forcing_channel, Case-Channel.f90:403-410), intt (AB3, or RK3
+    with st.hx = [f1, f2]), pre_correc with the wall branches, divergence,
+    Poisson, gradp storing the wall planes, cor_vel."""
     p = S.p
     st.itime += 1
     dt = p.dt
@@ -1753,6 +1755,8 @@ def step_channel(S: Schemes, st: State, wg: WallGrad, cfr=2.0 / 3.0):
         if cfr is not None:
             st.ux, _ = channel_cfr(S, st.ux, cfr)
         dux, duy, duz = momentum_rhs(S, st.ux, st.uy, st.uz)
+        if fcpg is not None:
+            dux = dux + fcpg
         st.hx[0], st.hy[0], st.hz[0] = dux, duy, duz
         if p.itimescheme == 5:
             st.ux = intt_rk3(st.ux, st.hx, itr, dt)

@@ -95,7 +95,7 @@ def test_host_stretching_equals_oracle(lib, istret, beta):
     kw = dict(nx=16, ny=65, nz=8, ncly1=2, nclyn=1, istret=istret, beta=beta)
     st = O.stretching(O.Params(**kw))
     xp = X.make_params(**kw)
-    for name in ("yp", "ypi", "ppy", "ppyi"):
+    for name in ("yp", "ypi", "ppy", "ppyi", "pp2y", "pp4y"):
         got, ref = X.schemes_host(xp, name), st[name]
         assert got.shape == ref.shape
         assert np.abs(got - ref).max() <= 4e-16 * np.abs(ref).max(), name

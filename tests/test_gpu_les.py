@@ -146,18 +146,21 @@ def test_mlp_training_reduces_loss(les32):
     m.close()
 
 
-def test_closure_matches_oracle(les32):
+@pytest.mark.parametrize("project", [0, 1])
+def test_closure_matches_oracle(les32, project):
+    """project 0: the network's stress (model 2); 1: its eddy-viscosity
+    projection (model 3), both with the Smagorinsky fallback."""
     import torch
     X, _, _ = les32
-    print("closure kernel", flush=True)
+    print(f"closure kernel project={project}", flush=True)
     m, params = _mlp(X, 13)
     rng = np.random.default_rng(14)
     n = 16
     g = {k: rng.standard_normal((n, n, n)) for k in L.GRAD_ORDER}
     gd = [torch.tensor(g[k], device="cuda") for k in L.GRAD_ORDER]
     gscale, delta, delta0, cs, beta = 1.3, 0.1, 0.1, 0.17, 0.0
-    t, nfb = m.closure(gd, gscale, delta, delta0, cs, beta)
-    ref, bad = L.closure_stress(params, g, delta, gscale, delta0, cs, beta)
+    t, nfb = m.closure(gd, gscale, delta, delta0, cs, beta, project)
+    ref, bad = L.closure_stress(params, g, delta, gscale, delta0, cs, beta, project=bool(project))
     # points where the fallback decision agrees
     s = L.strain(g)
     nut = (cs * delta) ** 2 * np.sqrt(2 * L.srt(s))

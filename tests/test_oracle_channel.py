@@ -99,3 +99,27 @@ def test_channel_step_wall_invariants(scheme):
         # RK3 rescales the stored gradient by gdt(itr)/gdt(itr-1) between
         # sub-steps: the slip stays bounded by its first value
         assert max(slips[1:]) <= slips[0], slips
+
+
+@pytest.mark.parametrize("scheme", [3, 5])
+def test_poiseuille_is_steady_under_its_pressure_gradient(scheme):
+    """Laminar Poiseuille u = 1 - y^2 with the constant pressure gradient
+    fcpg = 2 nu (cpg branch) is a steady state: nu u'' + fcpg = 0.  On a
+    uniform mesh the compact schemes and their wall closures differentiate
+    quadratics exactly, so every step returns the profile to roundoff; a
+    wrong sign or a missing factor on fcpg drives the flow off at O(dt)."""
+    re = 100.0
+    p, S = _channel(nx=16, ny=33, nz=8, istret=0, re=re, dt=0.01, itimescheme=scheme)
+    y = O.channel_y(S)
+    u0 = np.broadcast_to((1.0 - y * y)[None, :, None], (p.nz, p.ny, p.nx)).copy()
+    z = np.zeros_like(u0)
+    st = O.State(u0.copy(), z.copy(), z.copy(), [u0.copy()] * 3, [z.copy()] * 3, [z.copy()] * 3, 0)
+    wg = O.wall_grad_zero(p)
+    for _ in range(5):
+        st, wg = O.step_channel(S, st, wg, cfr=None, fcpg=2.0 / re)
+    assert np.abs(st.ux - u0).max() < 1e-12
+    assert np.abs(st.uy).max() < 1e-12 and np.abs(st.uz).max() < 1e-12
+    st2 = O.State(u0.copy(), z.copy(), z.copy(), [u0.copy()] * 3, [z.copy()] * 3, [z.copy()] * 3, 0)
+    wg2 = O.wall_grad_zero(p)
+    st2, wg2 = O.step_channel(S, st2, wg2, cfr=None, fcpg=-2.0 / re)
+    assert np.abs(st2.ux - u0).max() > 1e-5
