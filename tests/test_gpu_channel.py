@@ -115,7 +115,10 @@ def test_channel_256x129x128_invariants(precision):
             mx = max(float(t.abs().max()) for t in (ux, uy, uz))
             assert math.isfinite(mx) and mx < 2.0
             dmax, dmean = s.divergence_stats()
-            assert abs(dmax) < (1e-9 if precision == 0 else 1e-5)
+            # FP32 bulk: the FP64 divergence of the FP64 velocity carries the
+            # FP32 rounding of the pressure gradient (~6e-8) through the
+            # derivatives across the finest wall cells (dy ~ 3e-3)
+            assert abs(dmax) < (1e-9 if precision == 0 else 1e-4)
             # the flow rate is reset to 2/3 before every sub-step's right-hand side
             assert abs(s.bulk_velocity() - 2.0 / 3.0) < 2e-3
         s.close()
