@@ -209,6 +209,18 @@ int x3d_poisson_f32(float *rhs, const int *n1, const int *n2, const int *n3);
  * stream and the call returns at once.  precision 0 = FP64, 1 = FP32.
  * x3d_transpose_async stands for all four transposes (one GPU: a copy). */
 int x3d_poisson_async(int precision, void *rhs, int n1, int n2, int n3);
+/* x3d_poisson_async without allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels as x3d_poisson_async, with the spectral work array taken from a
+ * file-scope device array of 160 MiB (167772160 bytes) instead of a buffer
+ * the library allocates: it covers (nxm/2+1)*nym*nzm complex words of the
+ * precision, e.g. 256^3 in FP64 or 256x512x512 in FP32; larger grids return
+ * X3D_ERR_UNSUPPORTED.  The call allocates nothing (x3d_init must have run:
+ * it records the array's device address).  Every call on one GPU shares
+ * that GPU's array, so calls that overlap on one GPU (other streams, other
+ * host threads) race. */
+int x3d_poisson_async_noalloc(int precision, void *rhs, int n1, int n2, int n3);
 int x3d_transpose_async(int precision, const void *src, void *dst, int n1, int n2, int n3);
 
 /* ---- pencil transposes ----------------------------------------------------

@@ -117,6 +117,14 @@ def poisson_async(rhs) -> None:
           "poisson_async")
 
 
+def poisson_async_noalloc(rhs) -> None:
+    """x3d_poisson_async_noalloc: the enqueue-only solve with the library's
+    fixed device scratch (no allocation)."""
+    n1, n2, n3 = _dims(rhs)
+    check(_lib.lib().x3d_poisson_async_noalloc(0 if rhs.dtype == torch.float64 else 1, derive._dptr(rhs), n1,
+                                                n2, n3), "x3d_poisson_async_noalloc")
+
+
 def transpose_async(src, dst) -> None:
     """x3d_transpose_async: any of the four pencil transposes, queued only."""
     import torch

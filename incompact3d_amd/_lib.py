@@ -79,6 +79,7 @@ def _declare(lib):
     lib.x3d_line_op_async.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P, P, C.c_int, C.c_int,
                                       C.c_int]
     lib.x3d_poisson_async.argtypes = [C.c_int, P, C.c_int, C.c_int, C.c_int]
+    lib.x3d_poisson_async_noalloc.argtypes = [C.c_int, P, C.c_int, C.c_int, C.c_int]
     lib.x3d_transpose_async.argtypes = [C.c_int, P, P, C.c_int, C.c_int, C.c_int]
     lib.x3d_host_derx_00.argtypes = [P, P, P, P, P, P, P, IP, IP, IP, IP, P]
     lib.x3d_host_poisson.argtypes = [P, IP, IP, IP]

@@ -590,6 +590,26 @@ int x3d_poisson_async(int precision, void *rhs, int n1, int n2, int n3) {
   return fail(X3D_ERR_ARG, "precision must be 0 or 1");
 }
 
+int x3d_poisson_async_noalloc(int precision, void *rhs, int n1, int n2, int n3) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (precision != 0 && precision != 1) return fail(X3D_ERR_ARG, "precision must be 0 or 1");
+  if (n1 <= 0 || n2 <= 0 || n3 <= 0) return fail(X3D_ERR_ARG, "non-positive size");
+  if (!rhs) return fail(X3D_ERR_ARG, "null rhs");
+  if (!c->poisson) return fail(X3D_ERR_UNSUPPORTED, "no Poisson solver for these boundary conditions/sizes");
+  if (n1 != c->nxm || n2 != c->nym || n3 != c->nzm)
+    return fail(X3D_ERR_ARG, "rhs extents differ from the pressure grid (nxm, nym, nzm)");
+  const size_t need = precision == 0 ? poisson_work_bytes<double>(*c) : poisson_work_bytes<float>(*c);
+  if (need > POISSON_SCRATCH_BYTES)
+    return fail(X3D_ERR_UNSUPPORTED, "x3d_poisson_async_noalloc: the spectral work array needs " +
+                                         std::to_string(need) + " bytes, beyond the " +
+                                         std::to_string(POISSON_SCRATCH_BYTES) + "-byte scratch");
+  void *ptr = poisson_scratch(c->device);
+  if (!ptr) return fail(X3D_ERR_STATE, "no scratch address for this device (x3d_init not run on it)");
+  if (precision == 0) return poisson_run<double>(*c, static_cast<double *>(rhs), ptr, c->stream);
+  return poisson_run<float>(*c, static_cast<float *>(rhs), ptr, c->stream);
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------- transposes

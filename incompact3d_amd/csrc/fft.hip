@@ -972,7 +972,34 @@ __global__ void __launch_bounds__(64) penta_y_kernel(cplx<T> *__restrict__ cw, c
   }
 }
 
+// ---------------------------------------------------------------- scratch without allocation
+// The spectral work array of x3d_poisson_async_noalloc: a file-scope device
+// array of POISSON_SCRATCH_BYTES that every GPU the library runs on holds
+// until the process ends.  Its device address is taken once per device at
+// x3d_init (with that device current) and kept in g_scratch; host code uses
+// only that address, never the array's name.  Every call on one GPU shares
+// the array: calls that overlap on one GPU race.
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[POISSON_SCRATCH_BYTES];
+static void *g_scratch[MAX_DEVICES] = {};
+
+static int scratch_init(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return fail(X3D_ERR_UNSUPPORTED, "device ordinal beyond the scratch table");
+  if (g_scratch[device]) return X3D_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(X3D_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[device] = ptr;
+  return X3D_OK;
+}
+
+void *poisson_scratch(int device) {
+  return device >= 0 && device < MAX_DEVICES ? g_scratch[device] : nullptr;
+}
+
 int poisson_setup(Context &c) {
+  X3D_TRY(scratch_init(c.device));
   const DirCoef &X = c.dir[0], &Y = c.dir[1], &Z = c.dir[2];
   // dispatch of decomp_2d_poisson_init (poisson.f90:98-108): 000, 010 and
   // 11x (x and y non-periodic, z either); poisson_100 is not provided

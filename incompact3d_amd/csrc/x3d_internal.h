@@ -181,6 +181,10 @@ template <typename T>
 size_t poisson_work_bytes(const Context &c);
 template <typename T>
 int poisson_run(const Context &c, T *rhs, void *work, hipStream_t st);
+// device address of the file-scope scratch array on `device` (taken at
+// x3d_init), nullptr before; POISSON_SCRATCH_BYTES bytes
+constexpr size_t POISSON_SCRATCH_BYTES = (size_t)160 << 20;
+void *poisson_scratch(int device);
 
 // ---------------------------------------------------------------- fused sweeps
 // momentum.hip: the transeq right-hand side of component comp along dir
