@@ -6,8 +6,8 @@ Run in a container that has the reference checkout (reads text data only):
 
 Writes tests/golden/tgv_reference.json with
   * rows of examples/TGV-Taylor-Green-vortex/TGV_Re1600.dat (Incompact3d DNS
-    at 512^3): t, E_k, eps_t, eps, enstrophy at selected times, and the peak
-    of eps_t;
+    at 512^3): t, E_k, eps_t, eps, enstrophy at selected times, the peak
+    of eps_t, and the whole t, E_k, eps_t, eps history (t = 0..20);
   * the full tests/TGV-Taylor-Green-vortex/reference_time_evol.dat (CTest
     fixture: 65^3 free-slip RK3 run, columns t, E_k, eps, eps2, enstrophy)
     and its single-precision twin;
@@ -45,6 +45,8 @@ def main(ref):
                                "source": "examples/TGV-Taylor-Green-vortex/TGV_Re5000.dat"},
         "TGV_Re1600_dns512": {"columns": ["t", "E_k", "eps_t", "eps", "enstrophy"], "rows": sel,
                               "peak_eps_t": peak[:5],
+                              "history": [r[:4] for r in dns],
+                              "history_columns": ["t", "E_k", "eps_t", "eps"],
                               "source": "examples/TGV-Taylor-Green-vortex/TGV_Re1600.dat"},
         "ctest_tgv_freeslip_65": {"columns": ["t", "E_k", "eps", "eps2", "enstrophy"], "rows": ctest,
                                   "rows_single": ctest1,
