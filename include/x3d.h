@@ -126,6 +126,26 @@ int x3d_schemes_host(const x3d_params *p, const char *name, double *out, int cap
 X3D_DECL_DER_FAMILY(double, )
 X3D_DECL_DER_FAMILY(float, _f32)
 
+#define X3D_DECL_DER_FAMILY_HOST(T, SFX)                                         \
+  X3D_DECL_DER_X(x3d_host_derx_00##SFX, T) X3D_DECL_DER_X(x3d_host_derx_11##SFX, T)   \
+  X3D_DECL_DER_X(x3d_host_derx_12##SFX, T) X3D_DECL_DER_X(x3d_host_derx_21##SFX, T)   \
+  X3D_DECL_DER_X(x3d_host_derx_22##SFX, T)                                       \
+  X3D_DECL_DER_Y(x3d_host_dery_00##SFX, T) X3D_DECL_DER_Y(x3d_host_dery_11##SFX, T)   \
+  X3D_DECL_DER_Y(x3d_host_dery_12##SFX, T) X3D_DECL_DER_Y(x3d_host_dery_21##SFX, T)   \
+  X3D_DECL_DER_Y(x3d_host_dery_22##SFX, T)                                       \
+  X3D_DECL_DER_X(x3d_host_derz_00##SFX, T) X3D_DECL_DER_X(x3d_host_derz_11##SFX, T)   \
+  X3D_DECL_DER_X(x3d_host_derz_12##SFX, T) X3D_DECL_DER_X(x3d_host_derz_21##SFX, T)   \
+  X3D_DECL_DER_X(x3d_host_derz_22##SFX, T)                                       \
+  X3D_DECL_DER_X(x3d_host_derxx_00##SFX, T) X3D_DECL_DER_X(x3d_host_derxx_11##SFX, T) \
+  X3D_DECL_DER_X(x3d_host_derxx_12##SFX, T) X3D_DECL_DER_X(x3d_host_derxx_21##SFX, T) \
+  X3D_DECL_DER_X(x3d_host_derxx_22##SFX, T)                                      \
+  X3D_DECL_DER_X(x3d_host_deryy_00##SFX, T) X3D_DECL_DER_X(x3d_host_deryy_11##SFX, T) \
+  X3D_DECL_DER_X(x3d_host_deryy_12##SFX, T) X3D_DECL_DER_X(x3d_host_deryy_21##SFX, T) \
+  X3D_DECL_DER_X(x3d_host_deryy_22##SFX, T)                                      \
+  X3D_DECL_DER_X(x3d_host_derzz_00##SFX, T) X3D_DECL_DER_X(x3d_host_derzz_11##SFX, T) \
+  X3D_DECL_DER_X(x3d_host_derzz_12##SFX, T) X3D_DECL_DER_X(x3d_host_derzz_21##SFX, T) \
+  X3D_DECL_DER_X(x3d_host_derzz_22##SFX, T)
+
 /* ---- staggered operators (velocity grid n <-> pressure grid nm) ----------
  * Same argument order as the reference external subroutines:
  *   derxvp   derive.f90:3796  (tx,ux,rx,sx,cfx6,csx6,cwx6,nx,nxm,ny,nz,npaire)
@@ -135,49 +155,51 @@ X3D_DECL_DER_FAMILY(float, _f32)
  *   interyvp :4265 deryvp :4442 (+ppyi) interypv :4587 derypv :4775 (+ppy)
  *   derzvp :4920 interzvp :5105 derzpv :5287 interzpv :5426
  */
-#define X3D_DECL_STAG(T, SFX)                                                    \
-  int x3d_derxvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,    \
+#define X3D_DECL_STAG_P(P, T, SFX)                                                    \
+  int P##derxvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,    \
                       const T *cw, const int *nx, const int *nxm, const int *ny, \
                       const int *nz, const int *npaire);                         \
-  int x3d_interxvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,  \
+  int P##interxvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,  \
                         const T *cw, const int *nx, const int *nxm,              \
                         const int *ny, const int *nz, const int *npaire);        \
-  int x3d_derxpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1,  \
+  int P##derxpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1,  \
                       const T *cw1, const T *cf2, const T *cs2, const T *cw2,    \
                       const int *nxm, const int *nx, const int *ny,              \
                       const int *nz, const int *npaire);                         \
-  int x3d_interxpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1,              \
+  int P##interxpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1,              \
                         const T *cs1, const T *cw1, const T *cf2, const T *cs2,  \
                         const T *cw2, const int *nxm, const int *nx,             \
                         const int *ny, const int *nz, const int *npaire);        \
-  int x3d_interyvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,  \
+  int P##interyvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,  \
                         const T *cw, const int *nx, const int *ny,               \
                         const int *nym, const int *nz, const int *npaire);       \
-  int x3d_deryvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,    \
+  int P##deryvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,    \
                       const T *cw, const T *ppyi, const int *nx, const int *ny,  \
                       const int *nym, const int *nz, const int *npaire);         \
-  int x3d_interypv##SFX(T *t, const T *u, T *r, T *s, const T *cf1,              \
+  int P##interypv##SFX(T *t, const T *u, T *r, T *s, const T *cf1,              \
                         const T *cs1, const T *cw1, const T *cf2, const T *cs2,  \
                         const T *cw2, const int *nx, const int *nym,             \
                         const int *ny, const int *nz, const int *npaire);        \
-  int x3d_derypv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1,  \
+  int P##derypv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1,  \
                       const T *cw1, const T *cf2, const T *cs2, const T *cw2,    \
                       const T *ppy, const int *nx, const int *nym,               \
                       const int *ny, const int *nz, const int *npaire);          \
-  int x3d_derzvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,    \
+  int P##derzvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,    \
                       const T *cw, const int *nx, const int *ny, const int *nz,  \
                       const int *nzm, const int *npaire);                        \
-  int x3d_interzvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,  \
+  int P##interzvp##SFX(T *t, const T *u, T *r, T *s, const T *cf, const T *cs,  \
                         const T *cw, const int *nx, const int *ny,               \
                         const int *nz, const int *nzm, const int *npaire);       \
-  int x3d_derzpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1,  \
+  int P##derzpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1, const T *cs1,  \
                       const T *cw1, const T *cf2, const T *cs2, const T *cw2,    \
                       const int *nx, const int *ny, const int *nzm,              \
                       const int *nz, const int *npaire);                         \
-  int x3d_interzpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1,              \
+  int P##interzpv##SFX(T *t, const T *u, T *r, T *s, const T *cf1,              \
                         const T *cs1, const T *cw1, const T *cf2, const T *cs2,  \
                         const T *cw2, const int *nx, const int *ny,              \
                         const int *nzm, const int *nz, const int *npaire);
+#define X3D_DECL_STAG(T, SFX) X3D_DECL_STAG_P(x3d_, T, SFX)
+#define X3D_DECL_STAG_HOST(T, SFX) X3D_DECL_STAG_P(x3d_host_, T, SFX)
 
 X3D_DECL_STAG(double, )
 X3D_DECL_STAG(float, _f32)
@@ -223,6 +245,22 @@ int x3d_poisson_async(int precision, void *rhs, int n1, int n2, int n3);
 int x3d_poisson_async_noalloc(int precision, void *rhs, int n1, int n2, int n3);
 int x3d_transpose_async(int precision, const void *src, void *dst, int n1, int n2, int n3);
 
+/* ---- 3-D FFT ---------------------------------------------------------------
+ * decomp_2d_fft_3d of 2DECOMP&FFT v2.0.4 (external; called at
+ * poisson.f90:330, 405, 455, 638, 705, 994) after
+ * decomp_2d_fft_init(PHYSICAL_IN_Z): the unnormalised transform of the real
+ * pressure-grid field in (n1,n2,n3) = (nxm,nym,nzm) to its complex spectrum
+ * out (n1, n2, n3/2+1), i fastest, interleaved (re, im) as Fortran complex,
+ * with exp(-i k.x) (r2c) and back with exp(+i k.x) (c2r; c2r(r2c(f)) =
+ * n1 n2 n3 f).  The c2r input must be the spectrum of a real field (as the
+ * reference's is); it is not modified.  Device arrays, out of place,
+ * blocking.  Hand-written radix kernels, power-of-two extents as for the
+ * Poisson solver. */
+int x3d_fft3d_r2c(const double *in, void *out, const int *n1, const int *n2, const int *n3);
+int x3d_fft3d_c2r(const void *in, double *out, const int *n1, const int *n2, const int *n3);
+int x3d_fft3d_r2c_f32(const float *in, void *out, const int *n1, const int *n2, const int *n3);
+int x3d_fft3d_c2r_f32(const void *in, float *out, const int *n1, const int *n2, const int *n3);
+
 /* ---- pencil transposes ----------------------------------------------------
  * 2DECOMP&FFT v2.0.4 transpose_x_to_y / y_to_z / z_to_y / y_to_x (external;
  * call sites e.g. transeq.f90:163-165, navier.f90:316-318).  On one GPU every
@@ -242,11 +280,32 @@ int x3d_transpose_y_to_x_f32(const float *src, float *dst, const int *n1, const 
  * y fastest.  No counterpart in the reference (its pencils keep i fastest). */
 int x3d_reorder_ji(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
 
-/* ---- host-memory entries (arrays in host memory; staged through HBM) ---- */
-int x3d_host_derx_00(double *t, const double *u, double *r, double *s, const double *ff,
-                     const double *fs, const double *fw, const int *nx, const int *ny,
-                     const int *nz, const int *npaire, const double *lind);
+/* ---- host-memory entries -------------------------------------------------
+ * x3d_host_<name> takes exactly the arguments of x3d_<name> above, with the
+ * field arrays (t, u; rhs; in, out; src, dst) in host memory, as the
+ * reference's allocatables are (variables.f90:127-187): the input is copied
+ * to HBM, the device entry runs, the output is copied back.  For Fortran
+ * callers that keep their fields on the host; the PCIe copies make them the
+ * slow path (x3d_solver_* keeps everything resident).  The transposes are
+ * identity copies in host memory. */
+X3D_DECL_DER_FAMILY_HOST(double, )
+X3D_DECL_DER_FAMILY_HOST(float, _f32)
+X3D_DECL_STAG_HOST(double, )
+X3D_DECL_STAG_HOST(float, _f32)
 int x3d_host_poisson(double *rhs, const int *n1, const int *n2, const int *n3);
+int x3d_host_poisson_f32(float *rhs, const int *n1, const int *n2, const int *n3);
+int x3d_host_fft3d_r2c(const double *in, void *out, const int *n1, const int *n2, const int *n3);
+int x3d_host_fft3d_c2r(const void *in, double *out, const int *n1, const int *n2, const int *n3);
+int x3d_host_fft3d_r2c_f32(const float *in, void *out, const int *n1, const int *n2, const int *n3);
+int x3d_host_fft3d_c2r_f32(const void *in, float *out, const int *n1, const int *n2, const int *n3);
+int x3d_host_transpose_x_to_y(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
+int x3d_host_transpose_y_to_z(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
+int x3d_host_transpose_z_to_y(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
+int x3d_host_transpose_y_to_x(const double *src, double *dst, const int *n1, const int *n2, const int *n3);
+int x3d_host_transpose_x_to_y_f32(const float *src, float *dst, const int *n1, const int *n2, const int *n3);
+int x3d_host_transpose_y_to_z_f32(const float *src, float *dst, const int *n1, const int *n2, const int *n3);
+int x3d_host_transpose_z_to_y_f32(const float *src, float *dst, const int *n1, const int *n2, const int *n3);
+int x3d_host_transpose_y_to_x_f32(const float *src, float *dst, const int *n1, const int *n2, const int *n3);
 
 /* ---- device-resident time step (navier.f90 / transeq.f90 / time_integrators.f90)
  * A solver owns the velocity (ux,uy,uz), the AB3 history (dux1..3 of

@@ -117,6 +117,31 @@ def poisson_async(rhs) -> None:
           "poisson_async")
 
 
+def fft3d_r2c(f):
+    """decomp_2d_fft_3d(in, out) forward: f real (nzm, nym, nxm) device
+    tensor -> complex spectrum (nzm/2+1, nym, nxm) (the reference's
+    (nxm, nym, nzm/2+1) Fortran array), unnormalised, exp(-i k.x)."""
+    import torch
+    n1, n2, n3 = _dims(f)
+    cdt = torch.complex128 if f.dtype == torch.float64 else torch.complex64
+    out = torch.empty((n3 // 2 + 1, n2, n1), dtype=cdt, device=f.device)
+    fn = _lib.lib().x3d_fft3d_r2c if f.dtype == torch.float64 else _lib.lib().x3d_fft3d_r2c_f32
+    check(fn(derive._dptr(f), C.c_void_p(out.data_ptr()), iref(n1), iref(n2), iref(n3)), "fft3d_r2c")
+    return out
+
+
+def fft3d_c2r(spec, n3: int):
+    """decomp_2d_fft_3d(in, out) backward: the (n3/2+1, n2, n1) complex
+    spectrum of a real field -> real (n3, n2, n1), unnormalised, exp(+i k.x)."""
+    import torch
+    _, n2, n1 = spec.shape
+    rdt = torch.float64 if spec.dtype == torch.complex128 else torch.float32
+    out = torch.empty((n3, n2, n1), dtype=rdt, device=spec.device)
+    fn = _lib.lib().x3d_fft3d_c2r if rdt == torch.float64 else _lib.lib().x3d_fft3d_c2r_f32
+    check(fn(C.c_void_p(spec.data_ptr()), derive._dptr(out), iref(n1), iref(n2), iref(n3)), "fft3d_c2r")
+    return out
+
+
 def poisson_async_noalloc(rhs) -> None:
     """x3d_poisson_async_noalloc: the enqueue-only solve with the library's
     fixed device scratch (no allocation)."""

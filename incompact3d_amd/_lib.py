@@ -58,19 +58,25 @@ def _declare(lib):
     lib.x3d_get_coefficients.argtypes = [C.c_char_p, C.POINTER(C.c_double), C.c_int, IP]
     lib.x3d_get_scalar.argtypes = [C.c_char_p, C.POINTER(C.c_double)]
     lib.x3d_schemes_host.argtypes = [C.POINTER(Params), C.c_char_p, C.POINTER(C.c_double), C.c_int, IP]
-    for sfx in ("", "_f32"):
-        for d in ("x", "z", "xx", "yy", "zz"):
+    # device entries (x3d_) and their host-array twins (x3d_host_)
+    for pre in ("x3d_", "x3d_host_"):
+        for sfx in ("", "_f32"):
+            for d in ("x", "z", "xx", "yy", "zz"):
+                for bc in ("00", "11", "12", "21", "22"):
+                    getattr(lib, f"{pre}der{d}_{bc}{sfx}").argtypes = [P, P, P, P, P, P, P, IP, IP, IP, IP, P]
             for bc in ("00", "11", "12", "21", "22"):
-                getattr(lib, f"x3d_der{d}_{bc}{sfx}").argtypes = [P, P, P, P, P, P, P, IP, IP, IP, IP, P]
-        for bc in ("00", "11", "12", "21", "22"):
-            getattr(lib, f"x3d_dery_{bc}{sfx}").argtypes = [P, P, P, P, P, P, P, P, IP, IP, IP, IP, P]
-        for nm in ("derxvp", "interxvp", "interyvp", "derzvp", "interzvp"):
-            getattr(lib, f"x3d_{nm}{sfx}").argtypes = [P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
-        getattr(lib, f"x3d_deryvp{sfx}").argtypes = [P, P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
-        for nm in ("derxpv", "interxpv", "interypv", "derzpv", "interzpv"):
-            getattr(lib, f"x3d_{nm}{sfx}").argtypes = [P, P, P, P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
-        getattr(lib, f"x3d_derypv{sfx}").argtypes = [P, P, P, P, P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
-        getattr(lib, f"x3d_poisson{sfx}").argtypes = [P, IP, IP, IP]
+                getattr(lib, f"{pre}dery_{bc}{sfx}").argtypes = [P, P, P, P, P, P, P, P, IP, IP, IP, IP, P]
+            for nm in ("derxvp", "interxvp", "interyvp", "derzvp", "interzvp"):
+                getattr(lib, f"{pre}{nm}{sfx}").argtypes = [P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
+            getattr(lib, f"{pre}deryvp{sfx}").argtypes = [P, P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
+            for nm in ("derxpv", "interxpv", "interypv", "derzpv", "interzpv"):
+                getattr(lib, f"{pre}{nm}{sfx}").argtypes = [P, P, P, P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
+            getattr(lib, f"{pre}derypv{sfx}").argtypes = [P, P, P, P, P, P, P, P, P, P, P, IP, IP, IP, IP, IP]
+            getattr(lib, f"{pre}poisson{sfx}").argtypes = [P, IP, IP, IP]
+            for nm in ("fft3d_r2c", "fft3d_c2r"):
+                getattr(lib, f"{pre}{nm}{sfx}").argtypes = [P, P, IP, IP, IP]
+            for nm in ("transpose_x_to_y", "transpose_y_to_z", "transpose_z_to_y", "transpose_y_to_x"):
+                getattr(lib, f"{pre}{nm}{sfx}").argtypes = [P, P, IP, IP, IP]
     for nm in ("x3d_transpose_x_to_y", "x3d_transpose_y_to_z", "x3d_transpose_z_to_y",
                "x3d_transpose_y_to_x", "x3d_reorder_ji"):
         getattr(lib, nm).argtypes = [P, P, IP, IP, IP]
@@ -81,8 +87,6 @@ def _declare(lib):
     lib.x3d_poisson_async.argtypes = [C.c_int, P, C.c_int, C.c_int, C.c_int]
     lib.x3d_poisson_async_noalloc.argtypes = [C.c_int, P, C.c_int, C.c_int, C.c_int]
     lib.x3d_transpose_async.argtypes = [C.c_int, P, P, C.c_int, C.c_int, C.c_int]
-    lib.x3d_host_derx_00.argtypes = [P, P, P, P, P, P, P, IP, IP, IP, IP, P]
-    lib.x3d_host_poisson.argtypes = [P, IP, IP, IP]
     lib.x3d_solver_create.argtypes = [C.c_int, C.POINTER(P)]
     lib.x3d_solver_destroy.argtypes = [P]
     lib.x3d_solver_init_tgv.argtypes = [P]

@@ -590,6 +590,38 @@ int x3d_poisson_async(int precision, void *rhs, int n1, int n2, int n3) {
   return fail(X3D_ERR_ARG, "precision must be 0 or 1");
 }
 
+}  // extern "C"
+
+// decomp_2d_fft_3d (2DECOMP&FFT v2.0.4, physical space in z-pencils)
+template <typename T>
+static int fft3d_entry(bool forward, const void *in, void *out, const int *n1, const int *n2, const int *n3) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  X3D_TRY(check_dims(n1, n2, n3));
+  if (!in || !out) return fail(X3D_ERR_ARG, "null pointer");
+  if (in == out) return fail(X3D_ERR_ARG, "the 3-D FFT is out of place");
+  if (*n1 != c->nxm || *n2 != c->nym || *n3 != c->nzm)
+    return fail(X3D_ERR_ARG, "real extents differ from the pressure grid (nxm, nym, nzm)");
+  X3D_TRY(fft3d<T>(*c, forward, in, out, c->stream));
+  X3D_HIP(hipStreamSynchronize(c->stream));
+  return X3D_OK;
+}
+
+extern "C" {
+
+int x3d_fft3d_r2c(const double *in, void *out, const int *n1, const int *n2, const int *n3) {
+  return fft3d_entry<double>(true, in, out, n1, n2, n3);
+}
+int x3d_fft3d_c2r(const void *in, double *out, const int *n1, const int *n2, const int *n3) {
+  return fft3d_entry<double>(false, in, out, n1, n2, n3);
+}
+int x3d_fft3d_r2c_f32(const float *in, void *out, const int *n1, const int *n2, const int *n3) {
+  return fft3d_entry<float>(true, in, out, n1, n2, n3);
+}
+int x3d_fft3d_c2r_f32(const void *in, float *out, const int *n1, const int *n2, const int *n3) {
+  return fft3d_entry<float>(false, in, out, n1, n2, n3);
+}
+
 int x3d_poisson_async_noalloc(int precision, void *rhs, int n1, int n2, int n3) {
   Context *c;
   X3D_TRY(require_ctx(&c));
@@ -663,51 +695,6 @@ int x3d_reorder_ji(const double *src, double *dst, const int *n1, const int *n2,
   if (!src || !dst || src == dst) return fail(X3D_ERR_ARG, "bad pointers (reorder is out of place)");
   X3D_TRY(launch_reorder_ji<double>(src, dst, *n1, *n2, *n3, c->stream));
   X3D_HIP(hipStreamSynchronize(c->stream));
-  return X3D_OK;
-}
-
-// ---------------------------------------------------------------- host-memory entries
-static int stage(Context *c, size_t bytes, void **p) {
-  if (c->host_stage_bytes < bytes) {
-    if (c->host_stage) X3D_HIP(hipFree(c->host_stage));
-    c->host_stage = nullptr;
-    c->host_stage_bytes = 0;
-    X3D_HIP(hipMalloc(&c->host_stage, bytes));
-    c->host_stage_bytes = bytes;
-  }
-  *p = c->host_stage;
-  return X3D_OK;
-}
-
-int x3d_host_derx_00(double *t, const double *u, double *r, double *s, const double *ff, const double *fs,
-                     const double *fw, const int *nx, const int *ny, const int *nz, const int *npaire,
-                     const double *lind) {
-  Context *c;
-  X3D_TRY(require_ctx(&c));
-  X3D_TRY(check_dims(nx, ny, nz));
-  if (!t || !u) return fail(X3D_ERR_ARG, "null pointer");
-  const size_t n = (size_t)*nx * *ny * *nz;
-  void *p;
-  X3D_TRY(stage(c, 2 * n * sizeof(double), &p));
-  double *du = static_cast<double *>(p), *dt = du + n;
-  X3D_HIP(hipMemcpy(du, u, n * sizeof(double), hipMemcpyHostToDevice));
-  X3D_TRY(x3d_derx_00(dt, du, r, s, ff, fs, fw, nx, ny, nz, npaire, lind));
-  X3D_HIP(hipMemcpy(t, dt, n * sizeof(double), hipMemcpyDeviceToHost));
-  return X3D_OK;
-}
-
-int x3d_host_poisson(double *rhs, const int *n1, const int *n2, const int *n3) {
-  Context *c;
-  X3D_TRY(require_ctx(&c));
-  X3D_TRY(check_dims(n1, n2, n3));
-  if (!rhs) return fail(X3D_ERR_ARG, "null pointer");
-  const size_t n = (size_t)*n1 * *n2 * *n3;
-  void *p;
-  X3D_TRY(stage(c, n * sizeof(double), &p));
-  double *d = static_cast<double *>(p);
-  X3D_HIP(hipMemcpy(d, rhs, n * sizeof(double), hipMemcpyHostToDevice));
-  X3D_TRY(x3d_poisson(d, n1, n2, n3));
-  X3D_HIP(hipMemcpy(rhs, d, n * sizeof(double), hipMemcpyDeviceToHost));
   return X3D_OK;
 }
 

@@ -28,6 +28,7 @@
 // the outputs in the same cyclic order, so they are stored straight back;
 // the DCT reorders, the Poisson divide and the real-to-complex packing are
 // folded into those loads and stores.
+#include <algorithm>
 #include <cmath>
 
 #include "x3d_internal.h"
@@ -1202,5 +1203,84 @@ int poisson_solve(Context &c, T *rhs, hipStream_t st) {
 
 template int poisson_solve<double>(Context &, double *, hipStream_t);
 template int poisson_solve<float>(Context &, float *, hipStream_t);
+
+// ---------------------------------------------------------------- 3-D FFT (decomp_2d_fft_3d)
+// The unnormalised transform 2DECOMP's decomp_2d_fft_3d performs with
+// physical space in z-pencils (poisson.f90:325-330, 405): real (nx,ny,nz) to
+// complex (nx, ny, nz/2+1), i fastest, exp(-i k.x) forward and exp(+i k.x)
+// backward, on the pressure grid.  The passes are the Poisson solve's own
+// (r2c along x, c2c along y and z) on its half-x work array (nx/2+1, ny, nz);
+// one more pass maps between that layout and the half-z layout of the
+// reference through the symmetry of a real field's spectrum,
+// X(kx, ky, kz) = conj X(-kx, -ky, -kz).
+template <typename T>
+__global__ void __launch_bounds__(256) halfx_to_halfz(const cplx<T> *__restrict__ cw, cplx<T> *__restrict__ out,
+                                                      int nx, int ny, int nz, int nxh) {
+  const size_t total = (size_t)nx * ny * (nz / 2 + 1);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+    const int kx = (int)(q % nx);
+    const size_t r = q / nx;
+    const int ky = (int)(r % ny), kz = (int)(r / ny);
+    out[q] = kx < nxh ? cw[kx + (size_t)nxh * (ky + (size_t)ny * kz)]
+                      : cconj(cw[(nx - kx) + (size_t)nxh * ((ny - ky) % ny + (size_t)ny * ((nz - kz) % nz))]);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) halfz_to_halfx(const cplx<T> *__restrict__ in, cplx<T> *__restrict__ cw,
+                                                      int nx, int ny, int nz, int nxh) {
+  const int nzh = nz / 2 + 1;
+  const size_t total = (size_t)nxh * ny * nz;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+    const int kx = (int)(q % nxh);
+    const size_t r = q / nxh;
+    const int ky = (int)(r % ny), kz = (int)(r / ny);
+    cw[q] = kz < nzh ? in[kx + (size_t)nx * (ky + (size_t)ny * kz)]
+                     : cconj(in[(nx - kx) % nx + (size_t)nx * ((ny - ky) % ny + (size_t)ny * (nz - kz))]);
+  }
+}
+
+template <typename T>
+int fft3d(Context &c, bool forward, const void *in, void *out, hipStream_t st) {
+  auto *P = static_cast<PoissonPlan *>(c.poisson);
+  if (!P) return fail(X3D_ERR_STATE, "no FFT plan for this grid (see x3d_init)");
+  void **cwp = sizeof(T) == 8 ? &P->cw64 : &P->cw32;
+  if (!*cwp) X3D_HIP(hipMalloc(cwp, poisson_work_bytes<T>(c)));
+  cplx<T> *cw = static_cast<cplx<T> *>(*cwp);
+  // the plain transforms: no cosine pre/post-processing, whatever the
+  // boundaries the Poisson plan was built for
+  PoissonPlan Q = *P;
+  Q.xdct = Q.ydct = Q.zdct = 0;
+  Q.ystret = 0;
+  PoissonTables<T> tb;
+  if constexpr (sizeof(T) == 8) tb = P->t64;
+  else tb = P->t32;
+  const int nlines = Q.ny * Q.nz, M = Q.nx / 2;
+  const size_t nout = forward ? (size_t)Q.nx * Q.ny * (Q.nz / 2 + 1) : (size_t)Q.nxh * Q.ny * Q.nz;
+  const unsigned nb = (unsigned)std::min<size_t>((nout + 255) / 256, 65535u * 8u);
+  bool ok;
+  if (forward) {
+    ok = dispatch_x<T>(true, M, const_cast<T *>(static_cast<const T *>(in)), cw, nlines, 0, tb, st);
+    ok = ok && dispatch_s<T, FFT_Y_FWD>(Q.ny, Q, cw, tb, st);
+    ok = ok && dispatch_s<T, FFT_Z_FWD>(Q.nz, Q, cw, tb, st);
+    if (ok)
+      hipLaunchKernelGGL((halfx_to_halfz<T>), dim3(nb), dim3(256), 0, st, cw, static_cast<cplx<T> *>(out), Q.nx,
+                         Q.ny, Q.nz, Q.nxh);
+  } else {
+    hipLaunchKernelGGL((halfz_to_halfx<T>), dim3(nb), dim3(256), 0, st, static_cast<const cplx<T> *>(in), cw, Q.nx,
+                       Q.ny, Q.nz, Q.nxh);
+    ok = dispatch_s<T, FFT_Z_INV>(Q.nz, Q, cw, tb, st);
+    ok = ok && dispatch_s<T, FFT_Y_INV>(Q.ny, Q, cw, tb, st);
+    ok = ok && dispatch_x<T>(false, M, static_cast<T *>(out), cw, nlines, 0, tb, st);
+  }
+  if (!ok) return fail(X3D_ERR_UNSUPPORTED, "FFT: no kernel for this size");
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("fft3d launch: ") + hipGetErrorString(e));
+  return X3D_OK;
+}
+template int fft3d<double>(Context &, bool, const void *, void *, hipStream_t);
+template int fft3d<float>(Context &, bool, const void *, void *, hipStream_t);
 
 }  // namespace x3d

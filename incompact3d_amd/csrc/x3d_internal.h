@@ -183,6 +183,10 @@ template <typename T>
 int poisson_run(const Context &c, T *rhs, void *work, hipStream_t st);
 // device address of the file-scope scratch array on `device` (taken at
 // x3d_init), nullptr before; POISSON_SCRATCH_BYTES bytes
+// decomp_2d_fft_3d on the pressure grid: real (nx,ny,nz) <-> complex
+// (nx, ny, nz/2+1), unnormalised
+template <typename T>
+int fft3d(Context &c, bool forward, const void *in, void *out, hipStream_t st);
 constexpr size_t POISSON_SCRATCH_BYTES = (size_t)160 << 20;
 void *poisson_scratch(int device);
 

@@ -3,9 +3,11 @@ order, executed by libx3d on the GPU.
 
 Fields (t, u) are device tensors holding Fortran-ordered arrays, i.e. torch
 tensors of shape (nz, ny, nx) in C order (torch.float64 -> FP64 entry points,
-torch.float32 -> the ``_f32`` entry points).  Coefficient arrays (ffx, fsx,
-fwx, cfx6, ...) are host arrays, as the reference passes its module arrays;
-``r`` and ``s`` are the reference's scratch arguments and may be None.
+torch.float32 -> the ``_f32`` entry points), or numpy arrays of the same
+shape in host memory, which go to the ``x3d_host_*`` twins (staged through
+HBM).  Coefficient arrays (ffx, fsx, fwx, cfx6, ...) are host arrays, as the
+reference passes its module arrays; ``r`` and ``s`` are the reference's
+scratch arguments and may be None.
 
 Example, as in transeq.f90:125::
 
@@ -31,6 +33,26 @@ def _dptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _is_host(t):
+    return isinstance(t, np.ndarray)
+
+
+def _fptr(t, host):
+    """Pointer to a field: a C-contiguous numpy array for the host twins, a
+    device tensor otherwise (r and s may be None)."""
+    if t is None:
+        return None
+    if host:
+        if not isinstance(t, np.ndarray) or not t.flags.c_contiguous:
+            raise TypeError("host fields must be C-contiguous numpy arrays of shape (nz, ny, nx)")
+        return C.c_void_p(t.ctypes.data)
+    return _dptr(t)
+
+
+def _entry(name, sfx, host):
+    return getattr(_lib.lib(), f"x3d_{'host_' if host else ''}{name}{sfx}")
+
+
 class _HostArrays:
     """Keeps host coefficient arrays alive for the duration of one call."""
 
@@ -47,6 +69,12 @@ class _HostArrays:
 
 
 def _suffix(t):
+    if _is_host(t):
+        if t.dtype == np.float64:
+            return "", np.float64, C.c_double
+        if t.dtype == np.float32:
+            return "_f32", np.float32, C.c_float
+        raise TypeError("fields must be float64 or float32")
     import torch
     if t.dtype == torch.float64:
         return "", np.float64, C.c_double
@@ -63,9 +91,11 @@ def _der_x(name):
     def f(t, u, r, s, ff, fs, fw, nx, ny, nz, npaire, lind=0.0):
         sfx, dt, ct = _suffix(t)
         h = _HostArrays(dt)
-        fn = getattr(_lib.lib(), f"x3d_{name}{sfx}")
-        check(fn(_dptr(t), _dptr(u), _dptr(r), _dptr(s), h(ff), h(fs), h(fw), iref(nx), iref(ny),
-                 iref(nz), iref(npaire), _scalar_ref(lind, ct)), name)
+        host = _is_host(t)
+        fn = _entry(name, sfx, host)
+        fields = (_fptr(t, host), _fptr(u, host), _fptr(r, host), _fptr(s, host))
+        check(fn(*fields, h(ff), h(fs), h(fw), iref(nx), iref(ny), iref(nz), iref(npaire), _scalar_ref(lind, ct)),
+              name)
     f.__name__ = name
     f.__doc__ = f"{name}(t,u,r,s,ff,fs,fw,nx,ny,nz,npaire,lind) -- derive.f90 {name}"
     return f
@@ -75,9 +105,11 @@ def _der_y(name):
     def f(t, u, r, s, ff, fs, fw, pp, nx, ny, nz, npaire, lind=0.0):
         sfx, dt, ct = _suffix(t)
         h = _HostArrays(dt)
-        fn = getattr(_lib.lib(), f"x3d_{name}{sfx}")
-        check(fn(_dptr(t), _dptr(u), _dptr(r), _dptr(s), h(ff), h(fs), h(fw), h(pp), iref(nx),
-                 iref(ny), iref(nz), iref(npaire), _scalar_ref(lind, ct)), name)
+        host = _is_host(t)
+        fn = _entry(name, sfx, host)
+        fields = (_fptr(t, host), _fptr(u, host), _fptr(r, host), _fptr(s, host))
+        check(fn(*fields, h(ff), h(fs), h(fw), h(pp), iref(nx), iref(ny), iref(nz), iref(npaire),
+                 _scalar_ref(lind, ct)), name)
     f.__name__ = name
     f.__doc__ = f"{name}(t,u,r,s,ff,fs,fw,pp,nx,ny,nz,npaire,lind) -- derive.f90 {name}"
     return f
@@ -96,15 +128,16 @@ def _vp(name, with_pp=False):
     def f(t, u, r, s, cf, cs, cw, *rest):
         sfx, dt, _ = _suffix(t)
         h = _HostArrays(dt)
-        fn = getattr(_lib.lib(), f"x3d_{name}{sfx}")
+        host = _is_host(t)
+        fn = _entry(name, sfx, host)
         if with_pp:
             pp, n1, n2, n3, n4, npaire = rest
             args = [h(pp)]
         else:
             n1, n2, n3, n4, npaire = rest
             args = []
-        check(fn(_dptr(t), _dptr(u), _dptr(r), _dptr(s), h(cf), h(cs), h(cw), *args, iref(n1),
-                 iref(n2), iref(n3), iref(n4), iref(npaire)), name)
+        fields = (_fptr(t, host), _fptr(u, host), _fptr(r, host), _fptr(s, host))
+        check(fn(*fields, h(cf), h(cs), h(cw), *args, iref(n1), iref(n2), iref(n3), iref(n4), iref(npaire)), name)
     f.__name__ = name
     return f
 
@@ -114,15 +147,17 @@ def _pv(name, with_pp=False):
     def f(t, u, r, s, cf1, cs1, cw1, cf2, cs2, cw2, *rest):
         sfx, dt, _ = _suffix(t)
         h = _HostArrays(dt)
-        fn = getattr(_lib.lib(), f"x3d_{name}{sfx}")
+        host = _is_host(t)
+        fn = _entry(name, sfx, host)
         if with_pp:
             pp, n1, n2, n3, n4, npaire = rest
             args = [h(pp)]
         else:
             n1, n2, n3, n4, npaire = rest
             args = []
-        check(fn(_dptr(t), _dptr(u), _dptr(r), _dptr(s), h(cf1), h(cs1), h(cw1), h(cf2), h(cs2),
-                 h(cw2), *args, iref(n1), iref(n2), iref(n3), iref(n4), iref(npaire)), name)
+        fields = (_fptr(t, host), _fptr(u, host), _fptr(r, host), _fptr(s, host))
+        check(fn(*fields, h(cf1), h(cs1), h(cw1), h(cf2), h(cs2), h(cw2), *args, iref(n1), iref(n2), iref(n3),
+                 iref(n4), iref(npaire)), name)
     f.__name__ = name
     return f
 

@@ -3,6 +3,9 @@ against the CPU oracle, through the C-ABI mirror of the reference signatures.
 
 Bar: max|gpu - oracle| / max|oracle| <= 1e-12 in FP64,
 <= 2e-5 for the FP32 entries (FP32 storage and arithmetic).
+
+The same checks with host (numpy) fields, through the x3d_host_* twins, are
+in test_gpu_derive_host.py.
 """
 import numpy as np
 import pytest
@@ -39,20 +42,39 @@ CONFIGS = [
 ]
 
 
+def _context(param):
+    import incompact3d_amd as X
+    name, kw = param
+    p = O.Params(**kw)
+    X.init(X.make_params(**kw))
+    return name, p, O.Schemes(p), X
+
+
 @pytest.fixture(scope="module", params=CONFIGS, ids=[c[0] for c in CONFIGS])
 def ctx(request, torch_cuda):
-    import incompact3d_amd as X
-    name, kw = request.param
-    p = O.Params(**kw)
-    xp = X.make_params(**kw)
-    X.init(xp)
-    yield name, p, O.Schemes(p), X
-    X.finalize()
+    c = _context(request.param)
+    yield c
+    c[3].finalize()
 
 
 def _dev(a, dtype):
     import torch
     return torch.tensor(a, dtype=dtype, device="cuda")
+
+
+def _fields(mem, name, u, tdt, oshape=None):
+    """(input, output) fields in device or host memory."""
+    import torch
+    if mem == "host":
+        npdt = np.float64 if tdt == torch.float64 else np.float32
+        ud = np.ascontiguousarray(u, dtype=npdt)
+        return ud, np.empty(oshape if oshape else ud.shape, dtype=npdt)
+    ud = _dev(u, tdt)
+    return ud, torch.empty(oshape if oshape else ud.shape, dtype=tdt, device="cuda")
+
+
+def _result(td):
+    return td.astype(np.float64) if isinstance(td, np.ndarray) else td.double().cpu().numpy()
 
 
 def _bcname(S, ax):
@@ -65,6 +87,10 @@ def _bcname(S, ax):
 @pytest.mark.parametrize("kind", ["der1", "der2"])
 @pytest.mark.parametrize("npaire", [0, 1])
 def test_derivative(ctx, dtype, ax, kind, npaire):
+    _check_derivative(ctx, dtype, ax, kind, npaire, "device")
+
+
+def _check_derivative(ctx, dtype, ax, kind, npaire, mem):
     import torch
     from incompact3d_amd import derive as D
     name, p, S, X = ctx
@@ -81,14 +107,13 @@ def test_derivative(ctx, dtype, ax, kind, npaire):
         c = S.d2[ax]
         ff, fs, fw = (c["sfp"], c["ssp"], c["swp"]) if npaire == 1 else (c["sf"], c["ss"], c["sw"])
         fname = f"der{ax}{ax}_{_bcname(S, ax)}"
-    ud = _dev(u, tdt)
-    td = torch.empty_like(ud)
+    ud, td = _fields(mem, name, u, tdt)
     fn = getattr(D, fname)
     if kind == "der1" and ax == "y":
         fn(td, ud, None, None, ff, fs, fw, np.ones(p.ny), p.nx, p.ny, p.nz, npaire)
     else:
         fn(td, ud, None, None, ff, fs, fw, p.nx, p.ny, p.nz, npaire)
-    err = rel_err(td.double().cpu().numpy(), ref)
+    err = rel_err(_result(td), ref)
     assert err <= (TOL64 if dtype == "f64" else TOL32), f"{fname} {name} npaire={npaire}: {err:.3e}"
 
 
@@ -104,6 +129,10 @@ def _stag_case(S, p, op, ax, u_shape):
 @pytest.mark.parametrize("ax", ["x", "y", "z"])
 @pytest.mark.parametrize("op", ["dervp0", "dervp1", "intervp", "derpv", "interpv"])
 def test_staggered(ctx, dtype, ax, op):
+    _check_staggered(ctx, dtype, ax, op, "device")
+
+
+def _check_staggered(ctx, dtype, ax, op, mem):
     import torch
     from incompact3d_amd import derive as D
     name, p, S, X = ctx
@@ -123,8 +152,7 @@ def test_staggered(ctx, dtype, ax, op):
     out_dims = dict(in_dims)
     out_dims[ax] = nm if vp else n
     oshape = (out_dims["z"], out_dims["y"], out_dims["x"])
-    ud = _dev(u, tdt)
-    td = torch.empty(oshape, dtype=tdt, device="cuda")
+    ud, td = _fields(mem, name, u, tdt, oshape)
     nx, ny, nz = in_dims["x"], in_dims["y"], in_dims["z"]
     ones = np.ones(max(n, nm))
     if op in ("dervp0", "dervp1"):
@@ -164,7 +192,7 @@ def test_staggered(ctx, dtype, ax, op):
             D.interypv(td, ud, None, None, *args, nx, nm, n, nz, 1)
         else:
             D.interzpv(td, ud, None, None, *args, nx, ny, nm, n, 1)
-    err = rel_err(td.double().cpu().numpy(), ref)
+    err = rel_err(_result(td), ref)
     assert err <= (TOL64 if dtype == "f64" else TOL32), f"{op} {ax} {name}: {err:.3e}"
 
 
