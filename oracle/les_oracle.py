@@ -212,14 +212,20 @@ def bf16(x):
     return out
 
 
+# GELU of the closure network: the tanh form of Hendrycks & Gimpel (2016),
+# 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))), which differs from the
+# erf form z Phi(z) by at most 4.8e-4 (torch's gelu(approximate="tanh")).
+GELU_K = math.sqrt(2.0 / math.pi)
+GELU_A = 0.044715
+
+
 def gelu(z):
-    from scipy.special import erf
-    return 0.5 * z * (1.0 + erf(z / math.sqrt(2.0)))
+    return 0.5 * z * (1.0 + np.tanh(GELU_K * (z + GELU_A * z ** 3)))
 
 
 def gelu_grad(z):
-    from scipy.special import erf
-    return 0.5 * (1.0 + erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
+    th = np.tanh(GELU_K * (z + GELU_A * z ** 3))
+    return 0.5 * (1.0 + th) + 0.5 * z * (1.0 - th * th) * GELU_K * (1.0 + 3.0 * GELU_A * z * z)
 
 
 def init_params(seed=0):

@@ -175,3 +175,45 @@ def test_closure_matches_oracle(les32, project):
         err = np.abs(got[k] - ref[k])[agree]
         assert np.max(err) <= 2e-2 * scale and np.mean(err) <= 2e-3 * scale, (k, np.max(err), scale)
     m.close()
+
+
+@pytest.mark.parametrize("model", [2, 3])
+def test_learned_sgs_and_steps_match_oracle(les32, model):
+    """x3d_solver_sgs with the learned closure (model 2: the network's
+    stress; model 3: its eddy-viscosity projection) against the oracle's
+    learned_sgs, and four AB3 LES steps against its step_les.  The fallback
+    bound beta is set so high that no point falls back (the fallback is
+    checked at the closure level above); what remains is the network in
+    BF16 on MFMA against the oracle's FP64 evaluation of the same BF16
+    roundings, carried through the stress divergence."""
+    X, p, S = les32
+    print(f"learned sgs model={model}", flush=True)
+    m, params = _mlp(X, 17)
+    u = _perturbed_tgv(p, seed=3)
+    gscale, cs, beta = 1.0, 0.17, 1e6
+    delta0 = L.filter_width(p)
+    s = X.Solver(X.make_params(p.nx, p.ny, p.nz, re=p.re, dt=p.dt), 0)
+    s.set_velocity(*[_dev(c) for c in u])
+    s.set_les(model, cs=cs, beta=beta, mlp=m, gscale=gscale, delta0=delta0)
+    got = [t.cpu().numpy() for t in s.sgs()]
+    assert s.les_fallbacks() == 0
+    fn = L.learned_sgs(params, gscale, delta0, cs, beta, project=(model == 3))
+    ref = fn(S, *u)
+    scale = max(np.abs(r).max() for r in ref)
+    for g, r in zip(got, ref):
+        err = np.abs(g - r)
+        print(f"  force max err {err.max() / scale:.2e}, mean {err.mean() / scale:.2e} of {scale:.3e}")
+        assert err.max() <= 3e-2 * scale and err.mean() <= 3e-3 * scale
+    st = O.State(u[0].copy(), u[1].copy(), u[2].copy(), [u[0].copy()] * 3, [u[1].copy()] * 3,
+                 [u[2].copy()] * 3, 0)
+    for _ in range(4):
+        L.step_les(S, st, fn)
+    s.set_velocity(*[_dev(c) for c in u])
+    s.step(4)
+    vel = [t.cpu().numpy() for t in s.velocity()]
+    s.close()
+    m.close()
+    for g, r in zip(vel, (st.ux, st.uy, st.uz)):
+        e = rel_err(g, r)
+        print(f"  velocity after 4 steps rel err {e:.2e}")
+        assert e <= 1e-5

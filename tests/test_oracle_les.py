@@ -34,7 +34,7 @@ def _torch_forward(params, x):
     h = rnd(torch.tensor(x))
     for i in range(4):
         z = h @ rnd(ws[i]).T + bs[i]
-        h = rnd(F.gelu(z)) if i < 3 else z
+        h = rnd(F.gelu(z, approximate="tanh")) if i < 3 else z
     return h, ws, bs
 
 
@@ -60,6 +60,21 @@ def test_gelu_grad_finite_difference():
     h = 1e-6
     fd = (L.gelu(z + h) - L.gelu(z - h)) / (2 * h)
     assert np.max(np.abs(fd - L.gelu_grad(z))) < 1e-8
+
+
+def test_gelu_is_torch_tanh_gelu_and_close_to_erf_gelu():
+    """The activation is torch's gelu(approximate="tanh") to roundoff, and
+    within 4.8e-4 of the erf form z Phi(z) (a wrong constant in the tanh
+    argument moves either bound by orders of magnitude)."""
+    import torch
+    import torch.nn.functional as F
+    from scipy.special import erf
+    z = np.linspace(-12, 12, 24001)
+    ref = F.gelu(torch.tensor(z), approximate="tanh").numpy()
+    assert np.max(np.abs(L.gelu(z) - ref)) < 1e-14
+    exact = 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    dev = np.max(np.abs(L.gelu(z) - exact))
+    assert 4.0e-4 < dev < 4.8e-4
 
 
 def test_adam_matches_torch():
