@@ -117,6 +117,63 @@ __global__ void __launch_bounds__(1024) mom_yz_kernel(const MomArgs<T> A) {
   }
 }
 
+// ---------------------------------------------------------------- one line on one wavefront
+// The two coefficient sets of a line, V points per lane (lane k owns points
+// kV .. kV+V-1), held in registers.
+template <typename T, int V>
+struct LineCoef {
+  T fs1[V], fw1[V], ffw1[V], sm1[V], fs2[V], fw2[V], ffw2[V], sm2[V];
+  __device__ __forceinline__ void load(const MomArgs<T> &A, const int lane) {
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      const int e = lane * V + m;
+      fs1[m] = A.pk1[4 * e]; fw1[m] = A.pk1[4 * e + 1]; ffw1[m] = A.pk1[4 * e + 2]; sm1[m] = A.pk1[4 * e + 3];
+      fs2[m] = A.pk2[4 * e]; fw2[m] = A.pk2[4 * e + 1]; ffw2[m] = A.pk2[4 * e + 2]; sm2[m] = A.pk2[4 * e + 3];
+    }
+  }
+};
+
+// -1/2 D1(u_d u_c) - 1/2 u_d D1(u_c) + nu D2(u_c) of one periodic line held
+// by a wavefront: the halos come from the neighbour lanes (DPP), the three
+// solves are wave_solve's lane-parallel Thomas sweeps.
+template <typename T, int V>
+__device__ __forceinline__ void mom_line(const MomArgs<T> &A, const LineCoef<T, V> &C, const T (&ud)[V],
+                                         const T (&uc)[V], T (&r)[V], const int lane) {
+  {
+    T p[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) p[m] = ud[m] * uc[m];
+    T v[V + 4];
+    halo_window<T, V, 2, 2>(p, v, lane);
+    T w[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) w[m] = A.s.afi * (v[m + 3] - v[m + 1]) + A.s.bfi * (v[m + 4] - v[m]);
+    wave_solve<T, V>(w, C.fs1, C.fw1, C.ffw1, C.sm1, A.alfa1, A.rden1, lane);
+#pragma unroll
+    for (int m = 0; m < V; ++m) r[m] = T(-0.5) * w[m];
+  }
+  {
+    T v[V + 4];
+    halo_window<T, V, 2, 2>(uc, v, lane);
+    T w[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) w[m] = A.s.afi * (v[m + 3] - v[m + 1]) + A.s.bfi * (v[m + 4] - v[m]);
+    wave_solve<T, V>(w, C.fs1, C.fw1, C.ffw1, C.sm1, A.alfa1, A.rden1, lane);
+#pragma unroll
+    for (int m = 0; m < V; ++m) r[m] += T(-0.5) * (ud[m] * w[m]);
+  }
+  {
+    T v[V + 8];
+    halo_window<T, V, 4, 4>(uc, v, lane);
+    T w[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) w[m] = interior_row<OP_DER2, T, V + 8>(v, m + 4, A.s);
+    wave_solve<T, V>(w, C.fs2, C.fw2, C.ffw2, C.sm2, A.alfa2, A.rden2, lane);
+#pragma unroll
+    for (int m = 0; m < V; ++m) r[m] += A.xnu * w[m];
+  }
+}
+
 // ---------------------------------------------------------------- x sweep (one wavefront per line)
 template <typename T, int V>
 __global__ void __launch_bounds__(256) mom_x_kernel(const MomArgs<T> A, int nlines) {
@@ -125,13 +182,8 @@ __global__ void __launch_bounds__(256) mom_x_kernel(const MomArgs<T> A, int nlin
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const int n = A.n;
   const int e0 = lane * V;
-  T fs1[V], fw1[V], ffw1[V], sm1[V], fs2[V], fw2[V], ffw2[V], sm2[V];
-#pragma unroll
-  for (int m = 0; m < V; ++m) {
-    const int e = e0 + m;
-    fs1[m] = A.pk1[4 * e]; fw1[m] = A.pk1[4 * e + 1]; ffw1[m] = A.pk1[4 * e + 2]; sm1[m] = A.pk1[4 * e + 3];
-    fs2[m] = A.pk2[4 * e]; fw2[m] = A.pk2[4 * e + 1]; ffw2[m] = A.pk2[4 * e + 2]; sm2[m] = A.pk2[4 * e + 3];
-  }
+  LineCoef<T, V> C;
+  C.load(A, lane);
   // the next line's velocities are loaded before this line is solved, so
   // that their latency overlaps the solves (each wave loops over several lines)
   T pud[V], puc[V];
@@ -149,39 +201,7 @@ __global__ void __launch_bounds__(256) mom_x_kernel(const MomArgs<T> A, int nlin
       load_run<T, V>(A.uc + (long long)(ln + nwaves) * n + e0, puc);
     }
     T r[V];
-    {
-      T p[V];
-#pragma unroll
-      for (int m = 0; m < V; ++m) p[m] = ud[m] * uc[m];
-      T v[V + 4];
-      halo_window<T, V, 2, 2>(p, v, lane);
-      T w[V];
-#pragma unroll
-      for (int m = 0; m < V; ++m) w[m] = A.s.afi * (v[m + 3] - v[m + 1]) + A.s.bfi * (v[m + 4] - v[m]);
-      wave_solve<T, V>(w, fs1, fw1, ffw1, sm1, A.alfa1, A.rden1, lane);
-#pragma unroll
-      for (int m = 0; m < V; ++m) r[m] = T(-0.5) * w[m];
-    }
-    {
-      T v[V + 4];
-      halo_window<T, V, 2, 2>(uc, v, lane);
-      T w[V];
-#pragma unroll
-      for (int m = 0; m < V; ++m) w[m] = A.s.afi * (v[m + 3] - v[m + 1]) + A.s.bfi * (v[m + 4] - v[m]);
-      wave_solve<T, V>(w, fs1, fw1, ffw1, sm1, A.alfa1, A.rden1, lane);
-#pragma unroll
-      for (int m = 0; m < V; ++m) r[m] += T(-0.5) * (ud[m] * w[m]);
-    }
-    {
-      T v[V + 8];
-      halo_window<T, V, 4, 4>(uc, v, lane);
-      T w[V];
-#pragma unroll
-      for (int m = 0; m < V; ++m) w[m] = interior_row<OP_DER2, T, V + 8>(v, m + 4, A.s);
-      wave_solve<T, V>(w, fs2, fw2, ffw2, sm2, A.alfa2, A.rden2, lane);
-#pragma unroll
-      for (int m = 0; m < V; ++m) r[m] += A.xnu * w[m];
-    }
+    mom_line<T, V>(A, C, ud, uc, r, lane);
     // every epilogue input is read before the first store (the compiler
     // cannot move loads above stores to possibly aliasing arrays)
     T acc[V], f2[V], f3[V];
