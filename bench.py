@@ -285,23 +285,76 @@ def learned_les_comparison(X, stream, fp64_ms_256, n_dns):
     }
 
 
-def cpu_baseline(n, steps=1):
-    """The oracle (numpy, single thread) on a bounded sample of the same
-    workload: `steps` TGV time steps at n^3 after one untimed step."""
-    import numpy as np  # noqa: F401
+def cpu_baseline(n, steps=3):
+    """The CPU restatement of the step (oracle/cport: C++ with OpenMP on every
+    host thread the process is given, FP64, checked against the numpy oracle
+    to 1e-12 by tests/test_cport.py) on a bounded sample of the same
+    workload: `steps` TGV time steps at n^3 after one untimed step.  It
+    stands in for the reference's CPU build, which needs 2DECOMP&FFT (not in
+    the reference tree) and is not built here."""
+    import numpy as np
+    from oracle import cport_step
     from oracle import x3d_oracle as O
     p = O.Params(n, n, n, re=1600.0, dt=0.002)
     S = O.Schemes(p)
-    st = O.tgv_state(p)
-    O.step(S, st)
+    c = cport_step.CportStep(S)
+    u = [np.ascontiguousarray(a) for a in O.init_tgv(p)]
+    c.step(*u, nsteps=1)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        O.step(S, st)
+    c.step(*u, nsteps=steps)
     dt = time.perf_counter() - t0
-    return {"value": round(n ** 3 * steps / dt / 1e6, 4), "unit": "Mpoint-steps/s", "cores": 1,
+    threads = c.threads()
+    c.close()
+    return {"value": round(n ** 3 * steps / dt / 1e6, 4), "unit": "Mpoint-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle (numpy FP64 restatement) TGV {n}^3, {steps} AB3 step(s) after 1 warm-up step, "
-                      f"{dt:.1f} s"}
+            "sample": f"oracle/cport (C++/OpenMP FP64 restatement of the step, {threads} threads) TGV {n}^3, "
+                      f"{steps} AB3 steps after 1 warm-up step, {dt:.1f} s"}
+
+
+def c3_history(X, stream, n=256, nsteps=10000, every=10):
+    """Config 3 to its full length: TGV n^3, Re=1600, AB3 dt=0.002, 10,000
+    steps to t=20, FP64 and FP32 bulk, statistics every 10 steps; the FP32
+    dissipation error against FP64 over the whole history, and the FP64 run
+    against the reference's 512^3 DNS (tests/golden/tgv_reference.json)."""
+    import numpy as np
+    import torch
+    p = X.make_params(n, n, n, re=1600.0, dt=0.002)
+    X.init(p)
+    X.set_stream(stream.cuda_stream)
+    hist, wall = {}, {}
+    try:
+        with torch.cuda.stream(stream):
+            for prec in (0, 1):
+                t0 = time.perf_counter()
+                s = X.Solver(p, prec)
+                s.init_tgv()
+                rows = [(0.0,) + tuple(s.tgv_stats())]
+                for k in range(nsteps // every):
+                    s.step(every)
+                    rows.append(((k + 1) * every * 0.002,) + tuple(s.tgv_stats()))
+                s.close()
+                hist[prec] = np.array(rows)
+                wall[prec] = time.perf_counter() - t0
+    finally:
+        X.finalize()
+    h64, h32 = hist[0], hist[1]
+    with open(os.path.join(ROOT, "tests", "golden", "tgv_reference.json")) as f:
+        dns = np.array(json.load(f)["TGV_Re1600_dns512"]["history"])   # t, E_k, eps_t, eps
+    t, ek, eps = h64[:, 0], h64[:, 1], h64[:, 2]
+    i, j = int(np.argmax(eps)), int(np.argmax(dns[:, 3]))
+    epst = -(ek[2:] - ek[:-2]) / (t[2:] - t[:-2])
+    k = int(np.argmax(dns[:, 2]))
+    return {
+        "workload": f"TGV {n}^3 Re=1600, AB3 dt=0.002, {nsteps} steps to t={nsteps * 0.002:g}, statistics every "
+                    f"{every} steps",
+        "fp32_vs_fp64_eps_max_rel_err": float(np.max(np.abs(h32[:, 2] - eps) / eps)),
+        "fp32_vs_fp64_ke_max_rel_err": float(np.max(np.abs(h32[:, 1] - ek) / ek)),
+        "fp64_peak_eps": [round(float(t[i]), 3), float(eps[i])],
+        "dns512_peak_eps": [float(dns[j, 0]), float(dns[j, 3])],
+        "fp64_peak_eps_rel_err_vs_dns512": float(abs(eps[i] - dns[j, 3]) / dns[j, 3]),
+        "fp64_peak_minus_dEk_dt_rel_err_vs_dns512": float(abs(epst.max() - dns[k, 2]) / dns[k, 2]),
+        "wall_seconds": {"fp64": round(wall[0], 1), "fp32_bulk": round(wall[1], 1)},
+    }
 
 
 def main():
@@ -311,7 +364,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the full-length (t=20) FP64 / FP32-bulk accuracy run (about 40 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-les", action="store_true",
                     help="skip the learned-LES time-to-solution comparison (TGV Re=5000, about 40 s)")
@@ -398,6 +453,9 @@ def main():
     X.finalize()
     if not args.no_les and r64 is not None:
         out["learned_les_equal_error"] = learned_les_comparison(X, stream, r64["event_ms_per_step"], n)
+    if not args.no_c3 and r64 is not None and r32 is not None and n == 256:
+        c3 = c3_history(X, stream)
+        out["fp32_bulk"]["t20_accuracy"] = c3
     if args.micro_n > 0:
         out["microbench"] = [microbench(X, args.micro_n, False, stream), microbench(X, args.micro_n, True, stream)]
         out["stretched_poisson"] = stretched_poisson_bench(X, stream)

@@ -219,3 +219,36 @@ def test_nonperiodic_staggered_closed_forms(bc, ax):
         assert np.allclose(got, field(-kp * np.sin(q * xv), n), atol=1e-12 * kp), ("derpv", k)
         got = O.dervp(S, field(np.sin(q * xv), n), ax, 0)
         assert np.allclose(got, field(kp * np.cos(q * xp), n - 1), atol=1e-12 * kp), ("dervp", k)
+
+
+def test_hyperviscous_second_derivative_design_conditions():
+    """isecondder = 5 (schemes.f90:674-688, the channel configuration's
+    scheme) on periodic lines, pinned by the conditions it is designed to
+    meet rather than its coefficient formulas: the modified wavenumber of a
+    cosine mode, -lambda h^2, equals pi^2 (1 + nu0/nu) at the grid cutoff
+    w = pi and (2 pi/3)^2 (1 + c_nu nu0/nu) at w = 2 pi/3, and approaches
+    w^2 at small w with at least fourth-order error."""
+    n, L = 48, 2.0 * math.pi
+    nu0nu, cnu = 4.0, 0.44
+    p = O.Params(n, 8, 8, xlx=L, isecondder=5, nu0nu=nu0nu, cnu=cnu)
+    S = O.Schemes(p)
+    h = p.dx
+    x = np.arange(n) * h
+
+    def kpp(m):
+        u = np.broadcast_to(np.cos(m * x)[None, None, :], (8, 8, n)).copy()
+        d2 = O.der2(S, u, "x", 0)
+        lam = d2[0, 0] / u[0, 0]
+        ok = np.abs(u[0, 0]) > 0.5
+        return float(np.mean(-lam[ok] * h * h)), float(np.ptp(lam[ok]) * h * h)
+
+    k_pi, spread = kpp(n // 2)               # w = pi
+    assert spread < 1e-9 and abs(k_pi - math.pi ** 2 * (1 + nu0nu)) < 1e-10
+    k_23, spread = kpp(n // 3)               # w = 2 pi / 3
+    assert spread < 1e-9 and abs(k_23 - (2 * math.pi / 3) ** 2 * (1 + cnu * nu0nu)) < 1e-10
+    errs = []
+    for m in (1, 2):
+        w = m * h
+        k, _ = kpp(m)
+        errs.append(abs(k - w * w) / (w * w))
+    assert errs[0] < 1e-4 and errs[1] / errs[0] > 2 ** 4 * 0.9, errs
