@@ -285,7 +285,7 @@ def learned_les_comparison(X, stream, fp64_ms_256, n_dns):
     }
 
 
-def cpu_baseline(n, steps=3):
+def cpu_baseline(n, steps=20):
     """The CPU restatement of the step (oracle/cport: C++ with OpenMP on every
     host thread the process is given, FP64, checked against the numpy oracle
     to 1e-12 by tests/test_cport.py) on a bounded sample of the same
@@ -364,7 +364,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the full-length (t=20) FP64 / FP32-bulk accuracy run (about 40 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -22,8 +22,16 @@ Parity pins (each checked in ``tests/test_oracle_*.py``):
   modified wavenumbers / transfer functions of the periodic operators;
 * periodic derivative and staggered operators: pinned by the eigenfunction
   (single Fourier mode) closed forms to ~1e-14 and by dense-matrix solves;
-* wall/free-slip closures: pinned by dense solves of the banded system built
-  independently from the stencil and by polynomial exactness;
+* wall/free-slip closures, every 1/2 face pairing (``_11 _12 _21 _22``):
+  pinned by dense solves of the banded system built independently from
+  Lele's published scheme and by polynomial exactness; the staggered
+  operators on non-periodic lines by their periodic closed forms on cosine
+  (and sine) modes of the doubled box;
+* the hyperviscous second derivative (isecondder = 5): its interior row by
+  the scheme's five design conditions (coefficients solved independently),
+  its free-slip and Dirichlet rows by the dense restatement built on them;
+* channel (``step_channel``): the laminar Poiseuille profile is a fixed
+  point, and the constant-flow-rate shift holds the bulk velocity;
 * ``waves`` and ``poisson_000``: pinned by the algebraic identity
   p_hat = -f_hat/K (zero when K < 1e-16) computed with numpy FFTs, and by the
   known answer p = poisson(div(grad p_exact)) -> p_exact - mean;

@@ -252,3 +252,136 @@ def test_hyperviscous_second_derivative_design_conditions():
         k, _ = kpp(m)
         errs.append(abs(k - w * w) / (w * w))
     assert errs[0] < 1e-4 and errs[1] / errs[0] > 2 ** 4 * 0.9, errs
+
+
+def _hyperviscous_interior(nu0nu, cnu):
+    """Coefficients (alpha, c1..c4) of the isecondder = 5 interior row
+    alpha t(i-1) + t(i) + alpha t(i+1) = sum_m c_m (u(i+m) - 2 u(i) + u(i-m)) / h^2,
+    solved from its five design conditions (Lamballais, Fortune & Laizet,
+    J. Comput. Phys. 230, 2011): sixth-order accuracy (the w^2, w^4 and w^6
+    terms of the modified wavenumber), pi^2 (1 + nu0/nu) at w = pi and
+    (2 pi/3)^2 (1 + c_nu nu0/nu) at w = 2 pi/3.  The unknowns enter linearly
+    once the wavenumber relation is multiplied by 1 + 2 alpha cos w."""
+    m = np.arange(1, 5, dtype=float)
+    rows, rhs = [], []
+    # Taylor: sum c_m m^2 = 1 + 2 alpha, sum c_m m^4 / 12 = alpha, sum c_m m^6 / 360 = alpha / 12
+    rows.append(np.r_[-2.0, m ** 2]); rhs.append(1.0)
+    rows.append(np.r_[-1.0, m ** 4 / 12.0]); rhs.append(0.0)
+    rows.append(np.r_[-1.0 / 12.0, m ** 6 / 360.0]); rhs.append(0.0)
+    for w, k in ((math.pi, math.pi ** 2 * (1 + nu0nu)),
+                 (2 * math.pi / 3, (2 * math.pi / 3) ** 2 * (1 + cnu * nu0nu))):
+        # sum 2 c_m (1 - cos m w) = k (1 + 2 alpha cos w)
+        rows.append(np.r_[-2.0 * k * math.cos(w), 2.0 * (1.0 - np.cos(m * w))]); rhs.append(k)
+    sol = np.linalg.solve(np.array(rows), np.array(rhs))
+    return sol[0], sol[1:]
+
+
+def _dense_hyperviscous(n, h, bc1, bcn, npaire, al, c):
+    """A, B of the isecondder = 5 second derivative on a non-periodic line
+    (schemes.f90:694-856, derive.f90:1500-1976): the hyperviscous interior
+    row everywhere, mirrored with the field's parity at free-slip faces; at
+    a Dirichlet face Lele's alpha = 11 wall row, the fourth-order Pade row
+    next to it and Lele's sixth-order interior row on the third and fourth
+    points."""
+    s = 1.0 if npaire == 1 else -1.0
+    A = np.zeros((n, n))
+    B = np.zeros((n, n))
+
+    def put(M, i, j, v, par):
+        if j < 0:
+            M[i, -j] += par * v
+        elif j > n - 1:
+            M[i, 2 * (n - 1) - j] += par * v
+        else:
+            M[i, j] += v
+
+    for i in range(n):
+        put(A, i, i, 1.0, 1.0)
+        put(A, i, i - 1, al, s)
+        put(A, i, i + 1, al, s)
+        for k, cm in enumerate(c, start=1):
+            put(B, i, i, -2.0 * cm / h ** 2, 1.0)
+            put(B, i, i + k, cm / h ** 2, s)
+            put(B, i, i - k, cm / h ** 2, s)
+    for face, bc in ((0, bc1), (1, bcn)):
+        if bc != 2:
+            continue
+        sgn = 1 if face == 0 else -1
+        r0 = 0 if face == 0 else n - 1
+        inw = lambda k: r0 + sgn * k         # noqa: E731
+        for k in range(4):
+            A[inw(k), :] = 0.0
+            B[inw(k), :] = 0.0
+        A[inw(0), inw(0)], A[inw(0), inw(1)] = 1.0, 11.0
+        for k, v in enumerate((13.0, -27.0, 15.0, -1.0)):
+            B[inw(0), inw(k)] = v / h ** 2
+        A[inw(1), inw(0)], A[inw(1), inw(1)], A[inw(1), inw(2)] = 0.1, 1.0, 0.1
+        for k, v in enumerate((1.2, -2.4, 1.2)):
+            B[inw(1), inw(k)] = v / h ** 2
+        for r in (2, 3):
+            A[inw(r), inw(r - 1)], A[inw(r), inw(r)], A[inw(r), inw(r + 1)] = AL2, 1.0, AL2
+            for o, v in ((0, -2.0 * (A2 + B2)), (1, A2), (-1, A2), (2, B2), (-2, B2)):
+                B[inw(r), inw(r + o)] += v / h ** 2
+    return A, B
+
+
+def test_hyperviscous_interior_from_design_conditions_matches_oracle_periodic():
+    """The solved coefficients reproduce the oracle's periodic isecondder = 5
+    operator on every Fourier mode of a 24-point line."""
+    nu0nu, cnu = 4.0, 0.44
+    al, c = _hyperviscous_interior(nu0nu, cnu)
+    n = 24
+    p = O.Params(n, 4, 4, xlx=2 * math.pi, isecondder=5, nu0nu=nu0nu, cnu=cnu)
+    S = O.Schemes(p)
+    h = p.dx
+    x = np.arange(n) * h
+    for q in range(n // 2 + 1):
+        w = q * h
+        kpp = sum(2 * cm * (1 - math.cos(k * w)) for k, cm in enumerate(c, start=1)) / (1 + 2 * al * math.cos(w))
+        u = np.broadcast_to(np.cos(q * x)[None, None, :], (4, 4, n)).copy()
+        got = O.der2(S, u, "x", 0)
+        assert np.abs(got + kpp / h ** 2 * u).max() <= 1e-11 * max(1.0, kpp / h ** 2), q
+
+
+@pytest.mark.parametrize("bc", MIXED, ids=lambda b: f"{b[0]}{b[1]}")
+@pytest.mark.parametrize("npaire", [0, 1])
+@pytest.mark.parametrize("ax", ["x", "y", "z"])
+def test_hyperviscous_wall_closures_equal_dense_restatement(bc, npaire, ax):
+    """isecondder = 5 (the channel's scheme) with free-slip and Dirichlet
+    faces: the oracle's deriv-2 rows equal the dense restatement above."""
+    n, nu0nu, cnu = 21, 4.0, 0.44
+    kw = {"x": dict(nclx1=bc[0], nclxn=bc[1]), "y": dict(ncly1=bc[0], nclyn=bc[1]),
+          "z": dict(nclz1=bc[0], nclzn=bc[1])}[ax]
+    dims = {"x": (n, 5, 4), "y": (5, n, 4), "z": (5, 4, n)}[ax]
+    p = O.Params(*dims, xlx=1.3, yly=1.7, zlz=2.1, isecondder=5, nu0nu=nu0nu, cnu=cnu, **kw)
+    S = O.Schemes(p)
+    h = {"x": p.dx, "y": p.dy, "z": p.dz}[ax]
+    rng = np.random.default_rng(31 + npaire)
+    u = rng.standard_normal((p.nz, p.ny, p.nx))
+    if npaire == 0:
+        lines = np.moveaxis(u, O.AXIS[ax], 0)
+        if bc[0] == 1:
+            lines[0] = 0.0
+        if bc[1] == 1:
+            lines[-1] = 0.0
+    al, c = _hyperviscous_interior(nu0nu, cnu)
+    A, B = _dense_hyperviscous(n, h, bc[0], bc[1], npaire, al, c)
+    M = np.linalg.solve(A, B)
+    axis = O.AXIS[ax]
+    ref = np.moveaxis(np.tensordot(M, np.moveaxis(u, axis, 0), axes=(1, 0)), 0, axis)
+    got = O.der2(S, u, ax, npaire)
+    assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max(), (bc, npaire, ax)
+
+
+def test_hyperviscous_dense_restatement_planted_mistake_fails():
+    """A wrong sign on the Dirichlet face's third-row stencil, planted in the
+    restatement, is caught against the oracle."""
+    n = 21
+    p = O.Params(5, n, 4, yly=1.7, ncly1=2, nclyn=2, isecondder=5)
+    S = O.Schemes(p)
+    al, c = _hyperviscous_interior(p.nu0nu, p.cnu)
+    A, B = _dense_hyperviscous(n, p.dy, 2, 2, 1, al, c)
+    B[2, 0] = -B[2, 0]
+    u = np.random.default_rng(5).standard_normal((p.nz, p.ny, p.nx))
+    ref = np.moveaxis(np.tensordot(np.linalg.solve(A, B), np.moveaxis(u, 1, 0), axes=(1, 0)), 0, 1)
+    assert np.abs(O.der2(S, u, "y", 1) - ref).max() > 1e-6 * np.abs(ref).max()
