@@ -813,10 +813,14 @@ static void build_penta_factors(const Context &c, const PoissonPlan &P, const st
   const double xa0 = alpha / pi + 0.5 / beta / pi;
   const double xa1 = c.p.istret == 1 ? 1.0 / 4.0 / beta / pi : -(1.0 / 4.0 / beta / pi);   // 2 and 3: -
   const double xa0_2 = xa0 * xa0, xa1_2 = xa1 * xa1, xa01 = xa0 * xa1, xa0p1_2 = (xa0 + xa1) * (xa0 + xa1);
-  const size_t stride = (size_t)nz * nsys * nh * nxh;
+  // the z tables are symmetric (wave_1d copies mode nz-kz from mode kz), so
+  // the matrices of kz and nz-kz are equal: planes kz = 0 .. nz/2 are built
+  // and the kernel reads plane min(kz, nz-kz)
+  const int nzp = nz / 2 + 1;
+  const size_t stride = (size_t)nzp * nsys * nh * nxh;
   out.assign(5 * stride, 0.0);
   std::vector<double> cw(nh), a(5 * nh);
-  for (int kz = 0; kz < nz; ++kz)
+  for (int kz = 0; kz < nzp; ++kz)
     for (int par = 0; par < nsys; ++par)
       for (int kx = 0; kx < nxh; ++kx) {
         const double TX = tx[kx], TZ = tz[kz], TX2 = TX * TX, TZ2 = TZ * TZ, XK2 = kx2[kx], ZK2 = kz2[kz];
@@ -928,7 +932,8 @@ __global__ void __launch_bounds__(64) penta_y_kernel(cplx<T> *__restrict__ cw, c
   const int kz = blockIdx.y / nsys, par = blockIdx.y - kz * nsys;
   if (kx >= nxh) return;
   const int nh = ny / nsys, n = nh - 1;
-  const size_t line = ((size_t)blockIdx.y * nh) * nxh + kx;
+  const int kzp = kz <= nz - kz ? kz : nz - kz;   // factor plane: kz and nz-kz share one
+  const size_t line = ((size_t)(kzp * nsys + par) * nh) * nxh + kx;
   const double *F1 = pf + line, *F2 = pf + stride + line, *RP = pf + 2 * stride + line;
   const double *A1 = pf + 3 * stride + line, *B1 = pf + 4 * stride + line;
   auto at = [&](int m) -> cplx<T> & { return cw[((size_t)kz * ny + (nsys * m + par)) * nxh + kx]; };
@@ -1037,8 +1042,9 @@ int poisson_setup(Context &c) {
     tabs[1].resize(P->nxh);
   }
   if (c.p.istret != 0) {
-    // stretched y: poisson_010's pentadiagonal branch (istret = 3 uses
-    // inversion5_v2 on the whole line, not provided)
+    // stretched y: poisson_010's pentadiagonal branch; istret = 3 solves one
+    // system over all nym modes (inversion5_v2), istret = 1, 2 the even and
+    // odd modes apart (inversion5_v1)
     if (P->xdct || P->zdct || !P->ydct) {
       delete P;
       return fail(X3D_ERR_UNSUPPORTED, "Poisson on a stretched y mesh: poisson_010 (x and z periodic) only");

@@ -17,7 +17,11 @@
 // sweep writes the step's increment in FP32 and the new FP32 velocity, and the
 // gradient correction adds the increment to the FP64 velocity, so that the
 // FP64 state is read and written once per step.
-#include "sweeps.h"
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "lines_impl.h"
 
 namespace x3d {
 

@@ -13,7 +13,11 @@
 // overwrite inputs: a line never leaves its block (y/z) or its wavefront (x),
 // and every window is loaded before the first barrier, stores come after
 // the last one.
-#include "sweeps.h"
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "lines_impl.h"
 
 namespace x3d {
 
