@@ -224,6 +224,33 @@ def microbench(X, n, walls, stream, reps=10):
     return {"grid": [n, ny, n], "y": "Dirichlet walls (ncl=2)" if walls else "periodic", "ops": out}
 
 
+def channel_step_bench(X, stream, warmup=20, steps=200):
+    """The Re_tau 180 channel step (256 x 129 x 128, stretched y, RK3,
+    hyperviscous second derivative, constant flow rate, spin-up rotation on)
+    from the laminar profile with 10 % noise: HIP-event ms per step, FP64
+    and FP32 bulk.  The full 20,000-step run and its friction Reynolds
+    number are a GPU test of their own (tests/test_gpu_channel_c4.py)."""
+    from incompact3d_amd import channel
+    p = channel.params()
+    npts = p.nx * p.ny * p.nz
+    out = {"workload": "channel Re_tau 180, 256x129x128, istret 2, RK3 dt 0.005, per-operator step"}
+    X.init(p)
+    X.set_stream(stream.cuda_stream)
+    try:
+        for prec, lab in ((0, "fp64"), (1, "fp32_bulk")):
+            s = X.Solver(p, prec)
+            s.init_channel(0.1, 7)
+            s.set_forcing(1, 2.0 / 3.0)
+            s.set_rotation(0.12, 5000)
+            s.step(warmup)
+            ms = _events_time(stream, lambda: s.step(steps), 1) / steps
+            s.close()
+            out[lab] = {"ms_per_step": round(ms, 4), "mpoint_steps_per_s": round(npts / (ms * 1e-3) / 1e6, 1)}
+    finally:
+        X.finalize()
+    return out
+
+
 def stretched_poisson_bench(X, stream, reps=10):
     """Pressure Poisson solve on the channel grid 256 x 129 x 128 (walls in y,
     ncl=2), uniform (poisson_010, istret=0) and stretched (istret=2,
@@ -373,6 +400,7 @@ def main():
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the full-length (t=20) FP64 / FP32-bulk accuracy run (about 40 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="skip the channel step timing (a few seconds)")
     ap.add_argument("--no-les", action="store_true",
                     help="skip the learned-LES time-to-solution comparison (TGV Re=5000, about 40 s)")
     ap.add_argument("--micro-n", type=int, default=512,
@@ -464,6 +492,8 @@ def main():
     if args.micro_n > 0:
         out["microbench"] = [microbench(X, args.micro_n, False, stream), microbench(X, args.micro_n, True, stream)]
         out["stretched_poisson"] = stretched_poisson_bench(X, stream)
+    if not args.no_c4:
+        out["channel_c4"] = channel_step_bench(X, stream)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
         out["cpu_baseline"]["host"] = f"{os.cpu_count()} logical CPUs visible"

@@ -362,6 +362,13 @@ int x3d_solver_init_channel(x3d_solver *s, double init_noise, unsigned long long
  * reference); 2 constant pressure gradient, value added to the x right-hand
  * side (cpg, fcpg of Case-Channel.f90:403-410). */
 int x3d_solver_set_forcing(x3d_solver *s, int mode, double value);
+/* Spin-up rotation of the channel (momentum_forcing_channel,
+ * Case-Channel.f90:413-418; wrotation and spinup_time of the &Statistics
+ * namelist): while the step index (1 at the first step, as itime with
+ * ifirst = 1) is below spinup_time, every (sub-)step adds -wrotation * uy
+ * to the x and +wrotation * ux to the y right-hand side.  spinup_time 0
+ * turns it off.  Per-operator step only (bounded y, or RK3). */
+int x3d_solver_set_rotation(x3d_solver *s, double wrotation, int spinup_time);
 /* Bulk velocity ub = dy / (yly nx nz) sum ux / ppy of channel_cfr
  * (Case-Channel.f90:234-246), FP64. */
 int x3d_solver_bulk_velocity(x3d_solver *s, double *ub);

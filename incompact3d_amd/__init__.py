@@ -244,6 +244,11 @@ class Solver:
         """0 none, 1 constant flow rate (channel_cfr), 2 constant pressure gradient."""
         check(_lib.lib().x3d_solver_set_forcing(self._h, int(mode), float(value)), "set_forcing")
 
+    def set_rotation(self, wrotation: float, spinup_time: int):
+        """Spin-up rotation of momentum_forcing_channel (Case-Channel.f90:413-418):
+        while itime < spinup_time, dux -= wrotation uy, duy += wrotation ux."""
+        check(_lib.lib().x3d_solver_set_rotation(self._h, float(wrotation), int(spinup_time)), "set_rotation")
+
     def bulk_velocity(self) -> float:
         v = C.c_double()
         check(_lib.lib().x3d_solver_bulk_velocity(self._h, C.byref(v)), "bulk_velocity")

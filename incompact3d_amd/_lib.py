@@ -99,6 +99,7 @@ def _declare(lib):
     lib.x3d_solver_momentum_yz.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
     lib.x3d_solver_init_channel.argtypes = [P, C.c_double, C.c_ulonglong]
     lib.x3d_solver_set_forcing.argtypes = [P, C.c_int, C.c_double]
+    lib.x3d_solver_set_rotation.argtypes = [P, C.c_double, C.c_int]
     lib.x3d_solver_bulk_velocity.argtypes = [P, C.POINTER(C.c_double)]
     LLP = C.POINTER(C.c_longlong)
     lib.x3d_solver_set_les.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, P, C.c_double, C.c_double]

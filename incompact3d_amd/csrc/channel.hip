@@ -116,6 +116,14 @@ __global__ void __launch_bounds__(CH_THREADS) add_const_kernel(T *__restrict__ f
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) f[q] = (T)((double)f[q] + c);
 }
 
+// the spin-up rotation of momentum_forcing_channel (Case-Channel.f90:413-418):
+// f += a v, with a = -wrotation for dux (v = uy) and +wrotation for duy (v = ux)
+template <typename T>
+__global__ void __launch_bounds__(CH_THREADS) axpy_kernel(T *__restrict__ f, const T *__restrict__ v, T a, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) f[q] = f[q] + a * v[q];
+}
+
 // pre_correc on the y walls (navier.f90:616-690, no MHD, byx = byy = byz = 0):
 // ux = 0 + dpdx gdt, uy = 0, uz = 0 + dpdz gdt on plane j of each wall; the
 // stored planes are dpdx(i,k), dpdz(i,k) at [k*nx + i].
@@ -187,6 +195,15 @@ int launch_add_const(T *f, double c, size_t n, hipStream_t st) {
 }
 template int launch_add_const<double>(double *, double, size_t, hipStream_t);
 template int launch_add_const<float>(float *, double, size_t, hipStream_t);
+
+template <typename T>
+int launch_axpy(T *f, const T *v, double a, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL((axpy_kernel<T>), dim3(ch_blocks(n)), dim3(CH_THREADS), 0, st, f, v, (T)a, n);
+  X3D_HIP(hipGetLastError());
+  return X3D_OK;
+}
+template int launch_axpy<double>(double *, const double *, double, size_t, hipStream_t);
+template int launch_axpy<float>(float *, const float *, double, size_t, hipStream_t);
 
 int launch_wall_velocity(double *const u[3], float *const v[3], const double *dpdx, const double *dpdz, double gdt,
                          int nx, int ny, int nz, int j, hipStream_t st) {

@@ -51,6 +51,8 @@ int launch_bulk(const double *u, const double *ppy, int nx, int ny, int nz, doub
 int launch_cfr_shift(double *u, float *u32, const double *out2, size_t n, hipStream_t st);
 template <typename T>
 int launch_add_const(T *f, double c, size_t n, hipStream_t st);
+template <typename T>
+int launch_axpy(T *f, const T *v, double a, size_t n, hipStream_t st);
 int launch_wall_velocity(double *const u[3], float *const v[3], const double *dpdx, const double *dpdz, double gdt,
                          int nx, int ny, int nz, int j, hipStream_t st);
 template <typename T>
@@ -92,6 +94,10 @@ struct x3d_solver {
   // pressure gradient fvalue added to the x right-hand side (cpg, :403-410)
   int forcing = 0;
   double fvalue = 0.0;
+  // spin-up rotation of the channel (Case-Channel.f90:413-418): while
+  // itime < spinup_time, dux -= wrotation uy and duy += wrotation ux
+  double wrotation = 0.0;
+  int spinup_time = 0;
   double *ycoord = nullptr;    // wall-normal coordinate of init_channel
 };
 
@@ -343,11 +349,16 @@ int channel_cfr(const Context &c, x3d_solver *S) {
   return launch_cfr_shift(S->u64[0], S->u32[0], S->red, S->n, c.stream);
 }
 
-// momentum_forcing_channel with cpg (Case-Channel.f90:403-410): dux1 += fcpg
+// momentum_forcing_channel (Case-Channel.f90:396-420) on component comp's
+// right-hand side f1: the constant pressure gradient dux1 += fcpg (cpg,
+// :403-410), then the spin-up rotation while itime < spinup_time (:413-418),
+// dux1 -= wrotation uy1 and duy1 += wrotation ux1 with the sub-step's velocity
 template <typename T>
-int channel_cpg(const Context &c, x3d_solver *S, T *f1x) {
-  if (S->forcing != 2) return X3D_OK;
-  return launch_add_const<T>(f1x, S->fvalue, S->n, c.stream);
+int channel_forcing(const Context &c, x3d_solver *S, int comp, T *f1, const T *const u[3]) {
+  if (comp == 0 && S->forcing == 2) X3D_TRY(launch_add_const<T>(f1, S->fvalue, S->n, c.stream));
+  if (S->itime < S->spinup_time && comp < 2)
+    X3D_TRY(launch_axpy<T>(f1, u[1 - comp], comp == 0 ? -S->wrotation : S->wrotation, S->n, c.stream));
+  return X3D_OK;
 }
 
 // One RK3 step (itimescheme=5, xcompact3d.f90:46-87): three sub-steps of
@@ -378,7 +389,7 @@ int step_rk3(Context &c, x3d_solver *S) {
       T *f1 = (T *)S->hist[comp][S->slot[0]];
       X3D_TRY(momentum_component<T>(c, cu, comp, W[0], f1, nx, ny, nz));
       if (S->les) X3D_TRY(launch_add<T>(f1, sgs[comp], S->n, c.stream));
-      if (comp == 0) X3D_TRY(channel_cpg<T>(c, S, f1));
+      X3D_TRY(channel_forcing<T>(c, S, comp, f1, cu));
     }
     for (int comp = 0; comp < 3; ++comp) {
       const T *f1 = (T *)S->hist[comp][S->slot[0]];
@@ -455,7 +466,7 @@ int step_once(Context &c, x3d_solver *S) {
       T *f1 = (T *)S->hist[comp][S->slot[0]];
       X3D_TRY(momentum_component<T>(c, cu, comp, W[0], f1, nx, ny, nz));
       if (S->les) X3D_TRY(launch_add<T>(f1, sgs[comp], S->n, c.stream));
-      if (comp == 0) X3D_TRY(channel_cpg<T>(c, S, f1));
+      X3D_TRY(channel_forcing<T>(c, S, comp, f1, cu));
     }
     for (int comp = 0; comp < 3; ++comp) {
       const T *f1 = (T *)S->hist[comp][S->slot[0]];
@@ -832,6 +843,18 @@ int x3d_solver_set_forcing(x3d_solver *S, int mode, double value) {
     return fail(X3D_ERR_UNSUPPORTED, "the constant flow rate is defined for a y-bounded channel");
   S->forcing = mode;
   S->fvalue = value;
+  return X3D_OK;
+}
+
+int x3d_solver_set_rotation(x3d_solver *S, double wrotation, int spinup_time) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S) return fail(X3D_ERR_ARG, "null solver");
+  if (!std::isfinite(wrotation) || spinup_time < 0) return fail(X3D_ERR_ARG, "bad rotation arguments");
+  if (S->fused && spinup_time > 0)
+    return fail(X3D_ERR_UNSUPPORTED, "the spin-up rotation runs on the per-operator step (bounded y, or RK3)");
+  S->wrotation = wrotation;
+  S->spinup_time = spinup_time;
   return X3D_OK;
 }
 

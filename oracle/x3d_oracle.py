@@ -1743,14 +1743,16 @@ def ab3_gdt(dt):
     return adt + bdt + cdt
 
 
-def step_channel(S: Schemes, st: State, wg: WallGrad, cfr=2.0 / 3.0, fcpg=None):
+def step_channel(S: Schemes, st: State, wg: WallGrad, cfr=2.0 / 3.0, fcpg=None, wrotation=0.0, spinup_time=0):
     """One time step of the channel (xcompact3d.f90:46-87 with
     itype_channel): per sub-step boundary_conditions (the constant flow
     rate of channel_cfr when cfr is not None), the momentum right-hand side
-    (plus the constant pressure gradient fcpg on dux when fcpg is not None,
-    momentum_forcing_channel, Case-Channel.f90:403-410), intt (AB3, or RK3
-    with st.hx = [f1, f2]), pre_correc with the wall branches, divergence,
-    Poisson, gradp storing the wall planes, cor_vel."""
+    with momentum_forcing_channel (Case-Channel.f90:396-420: the constant
+    pressure gradient fcpg on dux when fcpg is not None, then, while
+    itime < spinup_time, the rotation dux -= wrotation uy, duy += wrotation
+    ux of the sub-step's velocity), intt (AB3, or RK3 with st.hx = [f1,
+    f2]), pre_correc with the wall branches, divergence, Poisson, gradp
+    storing the wall planes, cor_vel."""
     p = S.p
     st.itime += 1
     dt = p.dt
@@ -1765,6 +1767,9 @@ def step_channel(S: Schemes, st: State, wg: WallGrad, cfr=2.0 / 3.0, fcpg=None):
         dux, duy, duz = momentum_rhs(S, st.ux, st.uy, st.uz)
         if fcpg is not None:
             dux = dux + fcpg
+        if st.itime < spinup_time:
+            dux = dux - wrotation * st.uy
+            duy = duy + wrotation * st.ux
         st.hx[0], st.hy[0], st.hz[0] = dux, duy, duz
         if p.itimescheme == 5:
             st.ux = intt_rk3(st.ux, st.hx, itr, dt)

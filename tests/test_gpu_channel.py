@@ -5,7 +5,8 @@ branches of navier.f90, transeq.f90:337-370).
 
 * init_channel: the counter-based noise and the Poiseuille profile, 1e-14.
 * FP64 steps vs the oracle: 1e-11 (AB3 and RK3; uniform and stretched
-  meshes; constant flow rate or pressure gradient).
+  meshes; constant flow rate or pressure gradient; with and without the
+  spin-up rotation, which stops after its last step).
 * FP32 bulk vs the FP64 oracle: 2e-5 relative.
 * At the configuration's grid 256x129x128 (istret=2, beta=0.259, RK3,
   dt=0.005): the wall-normal velocity is exactly zero on the walls, the
@@ -59,6 +60,9 @@ CASES = [
     (16, 33, 16, 0, 5, "cfr"),
     (32, 17, 16, 1, 3, "cpg"),
     (16, 33, 16, 3, 5, "cfr"),
+    # spin-up rotation (wrotation 0.12) on steps 1 and 2 of 4 (itime < 3)
+    (16, 33, 16, 2, 5, "cfr+rot"),
+    (32, 17, 16, 1, 3, "cpg+rot"),
 ]
 
 
@@ -73,18 +77,21 @@ def test_channel_steps_vs_oracle(case, precision):
                  [u[2].copy()] * 3, 0)
     wg = O.wall_grad_zero(p)
     nsteps = 4
-    fcpg = 2.0 / 500.0 if forcing == "cpg" else None
-    cfr = 2.0 / 3.0 if forcing == "cfr" else None
+    fcpg = 2.0 / 500.0 if forcing.startswith("cpg") else None
+    cfr = 2.0 / 3.0 if forcing.startswith("cfr") else None
+    wrot, spinup = (0.12, 3) if forcing.endswith("+rot") else (0.0, 0)
     for _ in range(nsteps):
-        st, wg = O.step_channel(S, st, wg, cfr=cfr, fcpg=fcpg)
+        st, wg = O.step_channel(S, st, wg, cfr=cfr, fcpg=fcpg, wrotation=wrot, spinup_time=spinup)
     X.init(X.make_params(**kw))
     try:
         s = X.Solver(X.make_params(**kw), precision)
         s.set_velocity(*[dev_tensor(c) for c in u])
-        if forcing == "cfr":
+        if cfr is not None:
             s.set_forcing(1, 2.0 / 3.0)
         else:
             s.set_forcing(2, fcpg)
+        if spinup:
+            s.set_rotation(wrot, spinup)
         s.step(nsteps)
         got = [t.cpu().numpy() for t in s.velocity()]
         ub = s.bulk_velocity()

@@ -123,3 +123,32 @@ def test_poiseuille_is_steady_under_its_pressure_gradient(scheme):
     wg2 = O.wall_grad_zero(p)
     st2, wg2 = O.step_channel(S, st2, wg2, cfr=None, fcpg=-2.0 / re)
     assert np.abs(st2.ux - u0).max() > 1e-5
+
+
+def test_spinup_rotation_is_a_rotation_of_the_velocity(monkeypatch):
+    """momentum_forcing_channel's spin-up term (Case-Channel.f90:413-418)
+    with the transport terms switched off: the forcing is the horizontal
+    velocity turned by +90 degrees and scaled by wrotation, so it is
+    orthogonal to u at every point (it does no work), its magnitude is
+    wrotation |(ux, uy)|, its sense is positive (ux fy - uy fx =
+    wrotation (ux^2 + uy^2)), and it acts only while itime < spinup_time."""
+    p, S = _channel(re=500.0, dt=0.002, itimescheme=3)
+    u = O.init_channel(S, 0.125, 5)
+    zero = np.zeros_like(u[0])
+    monkeypatch.setattr(O, "momentum_rhs", lambda S_, ux, uy, uz: (zero.copy(), zero.copy(), zero.copy()))
+    w = 0.12
+
+    def first_rhs(spinup):
+        st = O.State(u[0].copy(), u[1].copy(), u[2].copy(), [zero] * 3, [zero] * 3, [zero] * 3, 0)
+        st, _ = O.step_channel(S, st, O.wall_grad_zero(p), cfr=None, wrotation=w, spinup_time=spinup)
+        return st.hx[0], st.hy[0], st.hz[0]
+
+    fx, fy, fz = first_rhs(2)
+    ux, uy, uz = u
+    scale = w * float(np.max(ux * ux + uy * uy))
+    assert np.abs(ux * fx + uy * fy + uz * fz).max() <= 1e-14 * scale
+    assert np.abs(np.hypot(fx, fy) - w * np.hypot(ux, uy)).max() <= 1e-14 * w * np.abs(ux).max()
+    assert np.abs((ux * fy - uy * fx) - w * (ux * ux + uy * uy)).max() <= 1e-14 * scale
+    assert np.abs(fz).max() == 0.0
+    for f in first_rhs(1):
+        assert np.abs(f).max() == 0.0
