@@ -37,7 +37,8 @@ enum x3d_status {
   X3D_ERR_ARG = 1,        /* bad argument (size, bc, pointer) */
   X3D_ERR_STATE = 2,      /* x3d_init not called, or called twice */
   X3D_ERR_HIP = 3,        /* a HIP runtime call failed */
-  X3D_ERR_UNSUPPORTED = 4 /* option the library does not implement */
+  X3D_ERR_UNSUPPORTED = 4, /* option the library does not implement */
+  X3D_ERR_IO = 5           /* a checkpoint file could not be written or read */
 };
 
 /* Subset of the reference's namelist parameters that the hot path reads
@@ -372,6 +373,23 @@ int x3d_solver_set_rotation(x3d_solver *s, double wrotation, int spinup_time);
 /* Bulk velocity ub = dy / (yly nx nz) sum ux / ppy of channel_cfr
  * (Case-Channel.f90:234-246), FP64. */
 int x3d_solver_bulk_velocity(x3d_solver *s, double *ub);
+/* restart(..., iresflg=1) of tools.f90:205-383: writes the checkpoint of
+ * the solver's state after its last step into <rundir>/checkpoint/, one file
+ * per variable with the reference's names (ux, uy, uz; dux-2, duy-2, duz-2,
+ * dux-3, duy-3, duz-3 for AB3; pp on the pressure grid), each the global
+ * array in Fortran order as native FP64 words (decomp_2d_write_one,
+ * reduce_prec=.false.), and <rundir>/restart.info with the &Time (tfield =
+ * t, itime) and &NumParam namelists in the reference's formats
+ * (tools.f90:343-373).  X3D_ERR_STATE before the first step. */
+int x3d_solver_checkpoint(x3d_solver *s, const char *rundir, double t);
+/* restart(..., iresflg=0) of tools.f90:384-500: reads what
+ * x3d_solver_checkpoint wrote into a solver of the same grid, scheme and
+ * precision: velocity, AB3 history, pressure, itime from the &Time namelist
+ * (its tfield returned in *t), and with walls in y the wall planes of the
+ * pressure gradient, reconstructed from pp as gradp does (tools.f90:493-500).
+ * The next step continues the run bit for bit.  X3D_ERR_IO for a missing
+ * file or one whose size does not match the grid. */
+int x3d_solver_restart(x3d_solver *s, const char *rundir, double *t);
 /* Instrumentation, no counterpart in the reference: enqueue nrep times the
  * six fused y/z momentum sweeps of one step (z then y for each component,
  * on the current velocity, into the solver's work buffer; the velocity and

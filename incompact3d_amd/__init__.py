@@ -13,6 +13,7 @@ package is the host-side mirror of the reference interface:
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -187,6 +188,7 @@ class Solver:
         check(_lib.lib().x3d_solver_create(int(precision), C.byref(self._h)), "x3d_solver_create")
         self.precision = precision
         self._shape = (p.nz, p.ny, p.nx)
+        self._dt = p.dt
 
     def close(self):
         if self._h:
@@ -248,6 +250,19 @@ class Solver:
         """Spin-up rotation of momentum_forcing_channel (Case-Channel.f90:413-418):
         while itime < spinup_time, dux -= wrotation uy, duy += wrotation ux."""
         check(_lib.lib().x3d_solver_set_rotation(self._h, float(wrotation), int(spinup_time)), "set_rotation")
+
+    def checkpoint(self, rundir: str, t: float = None):
+        """restart(iresflg=1) of tools.f90: <rundir>/checkpoint/{ux,uy,uz,dux-2,...,pp}
+        and <rundir>/restart.info; t defaults to itime * dt."""
+        if t is None:
+            t = self.itime * self._dt
+        check(_lib.lib().x3d_solver_checkpoint(self._h, os.fsencode(rundir), float(t)), "checkpoint")
+
+    def restart(self, rundir: str) -> float:
+        """restart(iresflg=0): the state written by checkpoint; returns tfield."""
+        t = C.c_double(0.0)
+        check(_lib.lib().x3d_solver_restart(self._h, os.fsencode(rundir), C.byref(t)), "restart")
+        return t.value
 
     def bulk_velocity(self) -> float:
         v = C.c_double()

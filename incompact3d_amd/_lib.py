@@ -100,6 +100,8 @@ def _declare(lib):
     lib.x3d_solver_init_channel.argtypes = [P, C.c_double, C.c_ulonglong]
     lib.x3d_solver_set_forcing.argtypes = [P, C.c_int, C.c_double]
     lib.x3d_solver_set_rotation.argtypes = [P, C.c_double, C.c_int]
+    lib.x3d_solver_checkpoint.argtypes = [P, C.c_char_p, C.c_double]
+    lib.x3d_solver_restart.argtypes = [P, C.c_char_p, C.POINTER(C.c_double)]
     lib.x3d_solver_bulk_velocity.argtypes = [P, C.POINTER(C.c_double)]
     LLP = C.POINTER(C.c_longlong)
     lib.x3d_solver_set_les.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, P, C.c_double, C.c_double]

@@ -18,10 +18,16 @@
 // cor_vel) and the Poisson divide and reductions run in FP64.  RK3
 // (itimescheme=5), free-slip faces, y walls and stretched y meshes run on
 // the per-operator path.
+#include <sys/stat.h>
+
+#include <cerrno>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <utility>
+#include <vector>
 
 #include "x3d_internal.h"
 
@@ -72,7 +78,9 @@ struct x3d_solver {
   float *u32b[3] = {nullptr, nullptr, nullptr};   // then the two sets swap)
   void *hist[3][3] = {};       // [component][slot], compute precision
   int slot[3] = {0, 1, 2};     // slot index of f1, f2, f3
-  void *w[4] = {};             // work fields, compute precision
+  void *w[5] = {};             // work fields, compute precision; after a step
+                               // w[0] holds its (last sub-step's) pressure
+  bool have_pressure = false;  // w[0] holds a pressure (a step has run)
   void *diag = nullptr;        // 12 fields for the TGV statistics
   double *w64[4] = {};         // FP64 work for the divergence check of precision 1
   double *partial = nullptr;   // reduction partials
@@ -261,19 +269,20 @@ int divergence(const Context &c, const T *const u[3], T *const W[4], int nx, int
 // walls its x and z components over gdt are kept for pre_correc
 // (navier.f90:456-479).
 template <typename T>
-int gradp_correct(const Context &c, x3d_solver *S, T *const u[3], T *const W[4], double gdt) {
+int gradp_correct(const Context &c, x3d_solver *S, T *const u[3], T *const W[5], double gdt, bool apply = true) {
   const int nx = S->nx, ny = S->ny, nz = S->nz, nxm = c.nxm, nym = c.nym, nzm = c.nzm;
   X3D_TRY(line<T>(c, 2, OP_INTERPV, 1, W[0], nullptr, W[1], nullptr, nullptr, 0, 1, nxm, nym, nzm));
   X3D_TRY(line<T>(c, 2, OP_DERPV, 1, W[0], nullptr, W[2], nullptr, nullptr, 0, 1, nxm, nym, nzm));
   X3D_TRY(line<T>(c, 1, OP_INTERPV, 1, W[1], nullptr, W[3], nullptr, nullptr, 0, 1, nxm, nym, nz));
-  X3D_TRY(line<T>(c, 1, OP_DERPV, 1, W[1], nullptr, W[0], nullptr, nullptr, 0, 1, nxm, nym, nz));
+  X3D_TRY(line<T>(c, 1, OP_DERPV, 1, W[1], nullptr, W[4], nullptr, nullptr, 0, 1, nxm, nym, nz));
   X3D_TRY(line<T>(c, 1, OP_INTERPV, 1, W[2], nullptr, W[1], nullptr, nullptr, 0, 1, nxm, nym, nz));
-  // px from W[3], py from W[0], pz from W[1]
-  const T *src[3] = {W[3], W[0], W[1]};
+  // px from W[3], py from W[4], pz from W[1]; the pressure W[0] is kept
+  const T *src[3] = {W[3], W[4], W[1]};
   const int ops[3] = {OP_DERPV, OP_INTERPV, OP_INTERPV};
   const bool walls = S->wall1 || S->walln;
   const size_t np = (size_t)nx * nz;
   for (int i = 0; i < 3; ++i) {
+    if (!apply && !(walls && i != 1)) continue;
     if (sizeof(T) == 8 && !walls) {
       X3D_TRY(line<T>(c, 0, ops[i], 1, src[i], nullptr, u[i], u[i], nullptr, 1.0, -1.0, nxm, ny, nz));
       continue;
@@ -285,6 +294,7 @@ int gradp_correct(const Context &c, x3d_solver *S, T *const u[3], T *const W[4],
       if (S->wall1) X3D_TRY(launch_wall_store<T>(W[2], S->wallg + q * np, gdt, nx, ny, nz, 0, c.stream));
       if (S->walln) X3D_TRY(launch_wall_store<T>(W[2], S->wallg + (2 + q) * np, gdt, nx, ny, nz, ny - 1, c.stream));
     }
+    if (!apply) continue;
     X3D_TRY(launch_corvel<T>(S->u64[i], W[2], sizeof(T) == 8 ? nullptr : S->u32[i], S->n, c.stream));
   }
   return X3D_OK;
@@ -304,6 +314,7 @@ int momentum_yz(Context &c, x3d_solver *S, int nrep, double *bytes) {
     fields += (comp == 1 ? 1 : 2) + 2;        // y: u_y (and u_c), acc, out
   }
   *bytes = fields / 6.0 * (double)S->n * sizeof(T);
+  S->have_pressure = false;   // the probe writes W[0]
   for (int r = 0; r < nrep; ++r) {
     for (int comp = 0; comp < 3; ++comp) {
       X3D_TRY(launch_momentum<T>(c, 2, comp, cu, nullptr, W0, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0,
@@ -371,7 +382,7 @@ int step_rk3(Context &c, x3d_solver *S) {
   const int nx = S->nx, ny = S->ny, nz = S->nz;
   T *u[3];
   for (int i = 0; i < 3; ++i) u[i] = sizeof(T) == 8 ? (T *)S->u64[i] : (T *)S->u32[i];
-  T *W[4] = {(T *)S->w[0], (T *)S->w[1], (T *)S->w[2], (T *)S->w[3]};
+  T *W[5] = {(T *)S->w[0], (T *)S->w[1], (T *)S->w[2], (T *)S->w[3], (T *)S->w[4]};
   S->itime += 1;
   const double dt = c.p.dt;
   const double adt[3] = {(8.0 / 15.0) * dt, (5.0 / 12.0) * dt, (3.0 / 4.0) * dt};
@@ -412,7 +423,7 @@ int step_once(Context &c, x3d_solver *S) {
   const int nx = S->nx, ny = S->ny, nz = S->nz;
   T *u[3];
   for (int i = 0; i < 3; ++i) u[i] = sizeof(T) == 8 ? (T *)S->u64[i] : (T *)S->u32[i];
-  T *W[4] = {(T *)S->w[0], (T *)S->w[1], (T *)S->w[2], (T *)S->w[3]};
+  T *W[5] = {(T *)S->w[0], (T *)S->w[1], (T *)S->w[2], (T *)S->w[3], (T *)S->w[4]};
   S->itime += 1;
   // AB3 coefficients (time_integrators.f90:85-100; variables.f90:1365-1375)
   const double dt = c.p.dt;
@@ -498,9 +509,9 @@ int step_once(Context &c, x3d_solver *S) {
     T *const gz_out[3] = {W[1], W[2], nullptr};
     X3D_TRY(launch_projection<T>(c, SW_GRAD_Z, gz_in, gz_out, nullptr, nullptr, nullptr, nx, ny, nz, c.stream));
     const T *const gy_in[3] = {W[1], W[2], nullptr};
-    T *const gy_out[3] = {W[3], W[0], W[1]};
+    T *const gy_out[3] = {W[3], W[4], W[1]};   // W[0] keeps the pressure
     X3D_TRY(launch_projection<T>(c, SW_GRAD_Y, gy_in, gy_out, nullptr, nullptr, nullptr, nx, ny, nz, c.stream));
-    const T *const gx_in[3] = {W[3], W[0], W[1]};
+    const T *const gx_in[3] = {W[3], W[4], W[1]};
     double *const u64[3] = {S->u64[0], S->u64[1], S->u64[2]};
     float *const u32[3] = {S->u32[0], S->u32[1], S->u32[2]};
     const float *const du[3] = {(const float *)S->u64b[0], (const float *)S->u64b[1], (const float *)S->u64b[2]};
@@ -572,7 +583,7 @@ int alloc_solver(Context &c, x3d_solver *S) {
   }
   const size_t np = (size_t)c.nxm * c.nym * c.nzm;
   const size_t nw = n > np ? n : np;
-  for (int i = 0; i < 4; ++i) X3D_HIP(hipMalloc(&S->w[i], nw * es));
+  for (int i = 0; i < 5; ++i) X3D_HIP(hipMalloc(&S->w[i], nw * es));
   X3D_HIP(hipMalloc(&S->partial, 8 * 1024 * sizeof(double)));
   X3D_HIP(hipMalloc(&S->red, 8 * sizeof(double)));
   if (S->wall1 || S->walln) X3D_HIP(hipMalloc(&S->wallg, 4 * (size_t)S->nx * S->nz * sizeof(double)));
@@ -589,10 +600,10 @@ void free_solver(x3d_solver *S) {
     for (int j = 0; j < 3; ++j)
       if (S->hist[i][j]) (void)hipFree(S->hist[i][j]);
   }
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < 5; ++i)
     if (S->w[i]) (void)hipFree(S->w[i]);
+  for (int i = 0; i < 4; ++i)
     if (S->w64[i]) (void)hipFree(S->w64[i]);
-  }
   if (S->diag) (void)hipFree(S->diag);
   if (S->lesw) (void)hipFree(S->lesw);
   if (S->nfb) (void)hipFree(S->nfb);
@@ -612,6 +623,7 @@ int reset_history(Context &c, x3d_solver *S) {
   S->slot[1] = 1;
   S->slot[2] = 2;
   S->itime = 0;
+  S->have_pressure = false;
   // dpdxy1 .. dpdzyn start at zero (the wall velocity of the first pre_correc)
   if (S->wallg) X3D_HIP(hipMemsetAsync(S->wallg, 0, 4 * (size_t)S->nx * S->nz * sizeof(double), c.stream));
   if (S->precision == 1)
@@ -714,6 +726,7 @@ int x3d_solver_step(x3d_solver *S, int nsteps) {
     } else {
       X3D_TRY(step_once<float>(*c, S));
     }
+    S->have_pressure = true;
   }
   return X3D_OK;
 }
@@ -873,6 +886,194 @@ int x3d_solver_bulk_velocity(x3d_solver *S, double *ub) {
 int x3d_solver_itime(x3d_solver *S, int *itime) {
   if (!S || !itime) return fail(X3D_ERR_ARG, "null pointer");
   *itime = S->itime;
+  return X3D_OK;
+}
+
+// ---------------------------------------------------------------- checkpoint / restart
+// restart() of tools.f90:205-514 for this solver's state: the checkpoint
+// directory <rundir>/checkpoint holds one file per variable the reference
+// writes (ux, uy, uz; dux-2 .. duz-2 and dux-3 .. duz-3 for AB3; pp on the
+// pressure grid), each the global array in Fortran order as native FP64
+// (decomp_2d_write_one with reduce_prec=.false.), and <rundir>/restart.info
+// the &Time / &NumParam namelists with the reference's formats
+// (tools.f90:343-373).  Reading restores the velocity, the AB3 history, the
+// pressure, itime (the &Time namelist) and, with y walls, the wall planes of
+// the pressure gradient that gradp reconstructs from pp (tools.f90:493-500).
+namespace {
+
+int write_raw(const std::string &path, const double *a, size_t n) {
+  FILE *f = std::fopen(path.c_str(), "wb");
+  if (!f) return fail(X3D_ERR_IO, "checkpoint: cannot create " + path + ": " + std::strerror(errno));
+  const size_t w = std::fwrite(a, sizeof(double), n, f);
+  const int cl = std::fclose(f);
+  if (w != n || cl != 0) return fail(X3D_ERR_IO, "checkpoint: short write to " + path);
+  return X3D_OK;
+}
+
+int read_raw(const std::string &path, double *a, size_t n) {
+  FILE *f = std::fopen(path.c_str(), "rb");
+  if (!f) return fail(X3D_ERR_IO, "restart: cannot open " + path + ": " + std::strerror(errno));
+  std::fseek(f, 0, SEEK_END);
+  const long sz = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (sz != (long)(n * sizeof(double))) {
+    std::fclose(f);
+    return fail(X3D_ERR_IO, "restart: " + path + " holds " + std::to_string(sz) + " bytes, the grid needs " +
+                                std::to_string(n * sizeof(double)));
+  }
+  const size_t r = std::fread(a, sizeof(double), n, f);
+  std::fclose(f);
+  if (r != n) return fail(X3D_ERR_IO, "restart: short read from " + path);
+  return X3D_OK;
+}
+
+// device field of compute precision es (8 or 4) <-> FP64 host buffer
+int field_to_host(const void *dev, size_t es, size_t n, std::vector<double> &h, hipStream_t st) {
+  h.resize(n);
+  if (es == 8) {
+    X3D_HIP(hipMemcpyAsync(h.data(), dev, n * 8, hipMemcpyDeviceToHost, st));
+    X3D_HIP(hipStreamSynchronize(st));
+  } else {
+    std::vector<float> t(n);
+    X3D_HIP(hipMemcpyAsync(t.data(), dev, n * 4, hipMemcpyDeviceToHost, st));
+    X3D_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < n; ++i) h[i] = t[i];
+  }
+  return X3D_OK;
+}
+
+int field_from_host(void *dev, size_t es, const std::vector<double> &h, hipStream_t st) {
+  const size_t n = h.size();
+  if (es == 8) {
+    X3D_HIP(hipMemcpyAsync(dev, h.data(), n * 8, hipMemcpyHostToDevice, st));
+    X3D_HIP(hipStreamSynchronize(st));
+  } else {
+    std::vector<float> t(n);
+    for (size_t i = 0; i < n; ++i) t[i] = (float)h[i];
+    X3D_HIP(hipMemcpyAsync(dev, t.data(), n * 4, hipMemcpyHostToDevice, st));
+    X3D_HIP(hipStreamSynchronize(st));
+  }
+  return X3D_OK;
+}
+
+const char *const VNAME[3] = {"ux", "uy", "uz"};
+const char *const DNAME[3] = {"dux", "duy", "duz"};
+
+}  // namespace
+
+int x3d_solver_checkpoint(x3d_solver *S, const char *rundir, double t) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || !rundir) return fail(X3D_ERR_ARG, "null pointer");
+  if (!S->have_pressure) return fail(X3D_ERR_STATE, "checkpoint: no step has run since the field was set");
+  const std::string dir = std::string(rundir) + "/checkpoint";
+  if (mkdir(dir.c_str(), 0755) != 0 && errno != EEXIST)
+    return fail(X3D_ERR_IO, "checkpoint: cannot create " + dir + ": " + std::strerror(errno));
+  const size_t es = S->precision == 0 ? 8 : 4;
+  std::vector<double> h;
+  X3D_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 3; ++i) {
+    X3D_TRY(field_to_host(S->u64[i], 8, S->n, h, c->stream));
+    X3D_TRY(write_raw(dir + "/" + VNAME[i], h.data(), S->n));
+  }
+  if (c->p.itimescheme == 3) {
+    // after a step f(n) sits in slot[1] and f(n-1) in slot[2]: dux1(:,:,:,2)
+    // and dux1(:,:,:,3) of time_integrators.f90:85-100
+    for (int lag = 2; lag <= 3; ++lag)
+      for (int i = 0; i < 3; ++i) {
+        X3D_TRY(field_to_host(S->hist[i][S->slot[lag - 1]], es, S->n, h, c->stream));
+        X3D_TRY(write_raw(dir + "/" + DNAME[i] + "-" + std::to_string(lag), h.data(), S->n));
+      }
+  }
+  const size_t np = (size_t)c->nxm * c->nym * c->nzm;
+  X3D_TRY(field_to_host(S->w[0], es, np, h, c->stream));
+  X3D_TRY(write_raw(dir + "/pp", h.data(), np));
+  const std::string info = std::string(rundir) + "/restart.info";
+  FILE *f = std::fopen(info.c_str(), "w");
+  if (!f) return fail(X3D_ERR_IO, "checkpoint: cannot create " + info + ": " + std::strerror(errno));
+  const char *bar = "!=========================";
+  std::fprintf(f, "%s\n&Time\n%s\n", bar, bar);
+  std::fprintf(f, "tfield=   %16.4f\n", t);
+  std::fprintf(f, "itime=    %16d\n", S->itime);
+  std::fprintf(f, "/End\n%s\n&NumParam\n%s\n", bar, bar);
+  std::fprintf(f, "nx=       %16d\nny=       %16d\nnz=       %16d\n", S->nx, S->ny, S->nz);
+  std::fprintf(f, "Lx=       %16.4f\nLy=       %16.4f\nLz=       %16.4f\n", c->p.xlx, c->p.yly, c->p.zlz);
+  std::fprintf(f, "istret=   %16d\n", c->p.istret);
+  std::fprintf(f, "beta=     %16.12f\n", c->p.beta);
+  std::fprintf(f, "iscalar=  %16d\nnumscalar=%16d\n", 0, 0);
+  std::fprintf(f, "itimescheme=%14d\n", c->p.itimescheme);
+  std::fprintf(f, "iimplicit=%16d\n", 0);
+  std::fprintf(f, "/End\n%s\n", bar);
+  if (std::fclose(f) != 0) return fail(X3D_ERR_IO, "checkpoint: cannot write " + info);
+  return X3D_OK;
+}
+
+int x3d_solver_restart(x3d_solver *S, const char *rundir, double *t) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || !rundir) return fail(X3D_ERR_ARG, "null pointer");
+  const std::string dir = std::string(rundir) + "/checkpoint";
+  const size_t es = S->precision == 0 ? 8 : 4;
+  // &Time namelist: tfield and itime (tools.f90:457-470)
+  const std::string info = std::string(rundir) + "/restart.info";
+  double tfield = 0.0;
+  int itime = -1;
+  if (FILE *f = std::fopen(info.c_str(), "r")) {
+    char line[256];
+    bool in_time = false;
+    while (std::fgets(line, sizeof line, f)) {
+      const char *q = line;
+      while (*q == ' ') ++q;
+      if (!std::strncmp(q, "&Time", 5)) in_time = true;
+      else if (!std::strncmp(q, "/End", 4)) in_time = false;
+      else if (in_time && !std::strncmp(q, "tfield=", 7)) tfield = std::strtod(q + 7, nullptr);
+      else if (in_time && !std::strncmp(q, "itime=", 6)) itime = (int)std::strtol(q + 6, nullptr, 10);
+    }
+    std::fclose(f);
+  } else {
+    return fail(X3D_ERR_IO, "restart: cannot open " + info + ": " + std::strerror(errno));
+  }
+  if (itime < 0) return fail(X3D_ERR_IO, "restart: " + info + " has no itime in its &Time namelist");
+  std::vector<double> h(S->n);
+  X3D_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 3; ++i) {
+    X3D_TRY(read_raw(dir + "/" + VNAME[i], h.data(), S->n));
+    X3D_TRY(field_from_host(S->u64[i], 8, h, c->stream));
+    if (S->precision == 1) X3D_TRY(launch_cast(S->u64[i], S->u32[i], S->n, c->stream));
+  }
+  S->slot[0] = 0;
+  S->slot[1] = 1;
+  S->slot[2] = 2;
+  if (c->p.itimescheme == 3)
+    for (int lag = 2; lag <= 3; ++lag)
+      for (int i = 0; i < 3; ++i) {
+        X3D_TRY(read_raw(dir + "/" + DNAME[i] + "-" + std::to_string(lag), h.data(), S->n));
+        X3D_TRY(field_from_host(S->hist[i][lag - 1], es, h, c->stream));
+      }
+  const size_t np = (size_t)c->nxm * c->nym * c->nzm;
+  std::vector<double> hp(np);
+  X3D_TRY(read_raw(dir + "/pp", hp.data(), np));
+  X3D_TRY(field_from_host(S->w[0], es, hp, c->stream));
+  S->itime = itime;
+  S->have_pressure = true;
+  if (S->wall1 || S->walln) {
+    // gradp on the restored pressure stores the wall planes over gdt of the
+    // last sub-step (itr = 3 for RK3, 1 for AB3)
+    const double dt = c->p.dt;
+    const double gdt = c->p.itimescheme == 5 ? (3.0 / 4.0) * dt + (-5.0 / 12.0) * dt
+                                             : (23.0 / 12.0) * dt + (-(16.0 / 12.0) * dt) + (5.0 / 12.0) * dt;
+    if (S->precision == 0) {
+      double *u[3] = {S->u64[0], S->u64[1], S->u64[2]};
+      double *W[5] = {(double *)S->w[0], (double *)S->w[1], (double *)S->w[2], (double *)S->w[3], (double *)S->w[4]};
+      X3D_TRY(gradp_correct<double>(*c, S, u, W, gdt, false));
+    } else {
+      float *u[3] = {S->u32[0], S->u32[1], S->u32[2]};
+      float *W[5] = {(float *)S->w[0], (float *)S->w[1], (float *)S->w[2], (float *)S->w[3], (float *)S->w[4]};
+      X3D_TRY(gradp_correct<float>(*c, S, u, W, gdt, false));
+    }
+  }
+  X3D_HIP(hipStreamSynchronize(c->stream));
+  if (t) *t = tfield;
   return X3D_OK;
 }
 
