@@ -218,9 +218,10 @@ int x3d_line_op_async(int dir, int op, int npaire, int precision, const void *u,
 
 /* ---- pressure Poisson -----------------------------------------------------
  * Replaces the procedure pointer `poisson` (poisson.f90:57-65) bound by
- * decomp_2d_poisson_init to poisson_000 (poisson.f90:298), poisson_010
- * (poisson.f90:665; with istret != 0 its matrice_refinement branch,
- * poisson.f90:846-895) or poisson_11x (poisson.f90:1019).  rhs is the
+ * decomp_2d_poisson_init to poisson_000 (poisson.f90:298), poisson_100
+ * (poisson.f90:413), poisson_010 (poisson.f90:665; with istret != 0 its
+ * matrice_refinement branch, poisson.f90:846-895) or poisson_11x
+ * (poisson.f90:1019).  rhs is the
  * device pressure-grid field (nxm,nym,nzm), overwritten with the solution,
  * exactly like poisson_xxx(rhs).  The _f32 entry keeps FP32 storage and FFT
  * arithmetic and divides by the modified wavenumber (or applies the

@@ -270,8 +270,9 @@ int x3d_init(const x3d_params *p) {
       return r;
     }
   }
-  // a grid without a Poisson solver (poisson_100, sizes the FFT passes do
-  // not cover) still gets the derivative entries
+  // a grid without a Poisson solver (a boundary combination the reference
+  // does not solve, sizes the FFT passes do not cover) still gets the
+  // derivative entries
   {
     int r = poisson_setup(*c);
     if (r != X3D_OK && r != X3D_ERR_UNSUPPORTED) {

@@ -65,7 +65,7 @@ struct PoissonTables {
 struct PoissonPlan {
   int nx, ny, nz, nxh;              // pressure-grid sizes (ny = nym when y is not periodic)
   int ydct;                         // 1 when y is non-periodic (poisson_010, poisson_11x)
-  int xdct, zdct;                   // 1 when x / z are non-periodic (poisson_11x)
+  int xdct, zdct;                   // 1 when x / z are non-periodic (poisson_11x; x alone: poisson_100)
   double scale;                     // 1/(nx*ny*nz)
   void *tab64 = nullptr, *tab32 = nullptr;
   PoissonTables<double> t64;
@@ -978,6 +978,43 @@ __global__ void __launch_bounds__(64) penta_y_kernel(cplx<T> *__restrict__ cw, c
   }
 }
 
+// ---------------------------------------------------------------- poisson_100
+// x non-periodic, y and z periodic (poisson.f90:413-661).  After the x
+// cosine transform the real part of entry kx is cosine mode kx and the
+// imaginary part mode nx-kx; the complex FFTs along y and z then mix the two.
+// Both parts are transforms of real fields, so they are the Hermitian and
+// anti-Hermitian halves over (ky, kz) -> (-ky, -kz): A = (D + conj D')/2,
+// iB = (D - conj D')/2, divided by -K of mode kx and of mode nx-kx (FP64,
+// zero where K < 1e-16).  One thread per pair of points, in place; the
+// partner's results are conj(A) and -conj(iB) with the same K (the tables
+// are symmetric in ky and kz).
+template <typename T>
+__global__ void __launch_bounds__(256) split_divide_kernel(cplx<T> *__restrict__ cw, int nxh, int ny, int nz,
+                                                           double scale, PoissonTables<T> tb) {
+  const size_t total = (size_t)nxh * ny * nz;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const int a = (int)(q % nxh);
+  const size_t r = q / nxh;
+  const int ky = (int)(r % ny), kz = (int)(r / ny);
+  const int kyp = ky ? ny - ky : 0, kzp = kz ? nz - kz : 0;
+  const size_t qp = ((size_t)kzp * ny + kyp) * nxh + a;
+  if (qp < q) return;   // the partner's thread writes both
+  const cplx<T> d = cw[q], dp = cw[qp];
+  const int am = 2 * (nxh - 1) - a;
+  const double tyv = tb.ty[ky], tzv = tb.tz[kz], ky2 = tb.ky2[ky], kz2 = tb.kz2[kz];
+  const double txr = tb.tx[a], kx2r = tb.kx2[a], txi = tb.tx[am], kx2i = tb.kx2[am];
+  const double yz2 = (tyv * tzv) * (tyv * tzv);
+  const double Kr = kx2r * yz2 + ky2 * (txr * tzv) * (txr * tzv) + kz2 * (txr * tyv) * (txr * tyv);
+  const double Ki = kx2i * yz2 + ky2 * (txi * tzv) * (txi * tzv) + kz2 * (txi * tyv) * (txi * tyv);
+  const double fr = Kr < 1.0e-16 ? 0.0 : -scale / Kr;
+  const double fi = Ki < 1.0e-16 ? 0.0 : -scale / Ki;
+  const double ar = 0.5 * ((double)d.x + (double)dp.x), ai = 0.5 * ((double)d.y - (double)dp.y);
+  const double br = 0.5 * ((double)d.x - (double)dp.x), bi = 0.5 * ((double)d.y + (double)dp.y);
+  cw[q] = cplx<T>{(T)(ar * fr + br * fi), (T)(ai * fr + bi * fi)};
+  if (qp != q) cw[qp] = cplx<T>{(T)(ar * fr - br * fi), (T)(-ai * fr + bi * fi)};
+}
+
 // ---------------------------------------------------------------- scratch without allocation
 // The spectral work array of x3d_poisson_async_noalloc: a file-scope device
 // array of POISSON_SCRATCH_BYTES that every GPU the library runs on holds
@@ -1007,10 +1044,11 @@ void *poisson_scratch(int device) {
 int poisson_setup(Context &c) {
   X3D_TRY(scratch_init(c.device));
   const DirCoef &X = c.dir[0], &Y = c.dir[1], &Z = c.dir[2];
-  // dispatch of decomp_2d_poisson_init (poisson.f90:98-108): 000, 010 and
-  // 11x (x and y non-periodic, z either); poisson_100 is not provided
-  if (X.bc1 != 0 && Y.bc1 == 0)
-    return fail(X3D_ERR_UNSUPPORTED, "Poisson: x non-periodic with y periodic (poisson_100) is not provided");
+  // dispatch of decomp_2d_poisson_init (poisson.f90:98-108): 000, 100 (x
+  // non-periodic, y and z periodic), 010 and 11x (x and y non-periodic, z
+  // either)
+  if (X.bc1 != 0 && Y.bc1 == 0 && Z.bc1 != 0)
+    return fail(X3D_ERR_UNSUPPORTED, "Poisson: x and z non-periodic with y periodic is not a reference case");
   if (X.bc1 == 0 && Z.bc1 != 0)
     return fail(X3D_ERR_UNSUPPORTED, "Poisson: z non-periodic needs x and y non-periodic (poisson_11x)");
   auto *P = new PoissonPlan();
@@ -1175,7 +1213,16 @@ int poisson_run(const Context &c, T *rhs, void *work, hipStream_t st) {
   // sizes were checked by poisson_setup: every dispatch below finds its case
   bool ok = dispatch_x<T>(true, M, rhs, cw, nlines, P->xdct, tb, st);
   ok = ok && dispatch_s<T, FFT_Y_FWD>(P->ny, *P, cw, tb, st);
-  if (P->ystret) {
+  if (P->xdct && !P->ydct) {
+    // poisson_100: forward z, the pairwise split and divide, inverse z
+    ok = ok && dispatch_s<T, FFT_Z_FWD>(P->nz, *P, cw, tb, st);
+    if (ok) {
+      const size_t total = (size_t)P->nxh * P->ny * P->nz;
+      hipLaunchKernelGGL((split_divide_kernel<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, cw,
+                         P->nxh, P->ny, P->nz, P->scale, tb);
+    }
+    ok = ok && dispatch_s<T, FFT_Z_INV>(P->nz, *P, cw, tb, st);
+  } else if (P->ystret) {
     // stretched y: z transform, pentadiagonal solves along y, inverse z
     ok = ok && dispatch_s<T, FFT_Z_FWD>(P->nz, *P, cw, tb, st);
     if (ok) {

@@ -1341,6 +1341,63 @@ def poisson_11x(S: Schemes, rhs):
     return _unreorder(out, 2)
 
 
+def waves_kxyz_100(S: Schemes):
+    """kxyz for bcx = 1, bcy = bcz = 0 (the bcz == 0 branch of
+    poisson.f90:1702-1740): real, on the half spectrum in z, shape
+    (nz/2+1, ny, nxm); x index i is cosine mode i (w = pi i / nxm)."""
+    p = S.p
+    kx2, Tx = _wave_1d(S.st["x"], p.dx, p.nx, p.nxm, p.xlx, 1)
+    ky2, Ty = _wave_1d(S.st["y"], p.dy, p.ny, p.nym, p.yly, 0)
+    kz2, Tz = _wave_1d(S.st["z"], p.dz, p.nz, p.nzm, p.zlz, 0)
+    nzh = p.nz // 2 + 1
+    X2 = kx2[None, None, :]; Y2 = ky2[None, :, None]; Z2 = kz2[:nzh, None, None]
+    TX = Tx[None, None, :]; TY = Ty[None, :, None]; TZ = Tz[:nzh, None, None]
+    return X2 * (TY * TZ) ** 2 + Y2 * (TX * TZ) ** 2 + Z2 * (TX * TY) ** 2
+
+
+def poisson_100(S: Schemes, rhs):
+    """Poisson with non-periodic x, periodic y and z (poisson.f90:413-661),
+    restated step by step on the pressure grid (nz, ny, nxm): even/odd
+    reorder along x, r2c along z and complex FFTs along y and x, the shift
+    factors in z and y (with the sign of the upper half in y), the x pairing
+    of entries i and nx-i, the divide by kxyz (real and imaginary parts,
+    zero where |K| < epsilon), the inverse post-processing, c2r and the
+    inverse reorder."""
+    p = S.p
+    nx, ny, nz = p.nxm, p.ny, p.nz
+    assert rhs.shape == (nz, ny, nx)
+    r = _reorder(rhs, 2)                                   # :431-443
+    cw = np.fft.rfft(r, axis=0)
+    cw = np.fft.fft(cw, axis=1)
+    cw = np.fft.fft(cw, axis=2)
+    cw = cw / float(nx) / float(ny) / float(nz)            # :455-456
+    nzh = nz // 2 + 1
+    ax_, bx_ = abxyz(nx, 1)
+    ay_, by_ = abxyz(ny, 0)
+    az_, bz_ = abxyz(nz, 0)
+    AZ = az_[:nzh, None, None]; BZ = bz_[:nzh, None, None]
+    AY = ay_[None, :, None]; BY = by_[None, :, None]
+    upper = (np.arange(ny) > ny // 2)[None, :, None]       # j > ny/2+1, 1-based
+    t1, t2 = cw.real, cw.imag
+    t1, t2 = t1 * BZ + t2 * AZ, t2 * BZ - t1 * AZ          # :473-486
+    t1, t2 = t1 * BY + t2 * AY, t2 * BY - t1 * AY          # :488-503
+    t1, t2 = np.where(upper, -t1, t1), np.where(upper, -t2, t2)
+    t1, t2 = _pair_fwd(t1, t2, ax_, bx_, 2)               # :505-527
+    K = waves_kxyz_100(S)
+    small = np.abs(K) < EPS_POISSON
+    c1 = np.where(small, 0.0, t1 / (-np.where(small, 1.0, K)))   # :541-563
+    c2 = np.where(small, 0.0, t2 / (-np.where(small, 1.0, K)))
+    t1, t2 = _pair_bwd(c1, c2, ax_, bx_, 2)               # :567-587
+    t1, t2 = t1 * BY - t2 * AY, t2 * BY + t1 * AY          # :601-614
+    t1, t2 = np.where(upper, -t1, t1), np.where(upper, -t2, t2)
+    t1, t2 = t1 * BZ - t2 * AZ, t2 * BZ + t1 * AZ          # :617-628
+    cw = t1 + 1j * t2
+    cw = np.fft.ifft(cw, axis=2) * nx
+    cw = np.fft.ifft(cw, axis=1) * ny
+    out = np.fft.irfft(cw, n=nz, axis=0) * nz
+    return _unreorder(out, 2)                              # :640-650
+
+
 def poisson(S: Schemes, rhs):
     """Dispatch as decomp_2d_poisson_init (poisson.f90:98-108)."""
     p = S.p
@@ -1348,9 +1405,11 @@ def poisson(S: Schemes, rhs):
         return poisson_000(S, rhs)
     if p.nclx and (not p.ncly) and p.nclz:
         return poisson_010(S, rhs)
+    if (not p.nclx) and p.ncly and p.nclz:
+        return poisson_100(S, rhs)
     if (not p.nclx) and (not p.ncly):
         return poisson_11x(S, rhs)
-    raise NotImplementedError("poisson_100 is not restated")
+    raise NotImplementedError("boundary combination not supported (poisson.f90:98-108)")
 
 
 # ---------------------------------------------------------------------------

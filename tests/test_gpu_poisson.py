@@ -1,5 +1,5 @@
-"""GPU parity of the pressure Poisson solver (poisson_000 / poisson_010 /
-poisson_11x with z free-slip or periodic) and
+"""GPU parity of the pressure Poisson solver (poisson_000 / poisson_100 /
+poisson_010 / poisson_11x with z free-slip or periodic) and
 of the pencil transposes against the oracle, through the C-ABI.
 
 Bar: 1e-12 relative (FP64); the FP32 entry (FP32 FFTs, FP64 divide) is
@@ -23,6 +23,11 @@ CASES = [
     ("p11x_bcz1_33x17x65", dict(nx=33, ny=17, nz=65, nclx1=1, nclxn=1, ncly1=1, nclyn=1, nclz1=1, nclzn=1)),
     ("p11x_bcz0_65x33x32", dict(nx=65, ny=33, nz=32, nclx1=1, nclxn=1, ncly1=1, nclyn=1)),
     ("p11x_bcz1_65", dict(nx=65, ny=65, nz=65, nclx1=1, nclxn=1, ncly1=1, nclyn=1, nclz1=1, nclzn=1)),
+    # poisson_100: x non-periodic (free-slip or walls), y and z periodic
+    ("p100_33x16x16", dict(nx=33, ny=16, nz=16, nclx1=1, nclxn=1)),
+    ("p100_65x32x8", dict(nx=65, ny=32, nz=8, nclx1=2, nclxn=2)),
+    ("p100_17x8x64", dict(nx=17, ny=8, nz=64, nclx1=1, nclxn=1)),
+    ("p100_129x16x32", dict(nx=129, ny=16, nz=32, nclx1=2, nclxn=1)),
     # the benchmark's grid and the 256- and 512-point passes of every
     # direction (x: r2c/c2r of 128 and 256 complex points; y, z: the strided
     # passes of 256 and 512 points, the z pass with the divide)

@@ -231,6 +231,49 @@ def test_poisson_11x_inverts_freeslip_div_grad():
     assert np.abs(pr - (pe - pe.mean())).max() <= 1e-13 * np.abs(pe).max()
 
 
+@pytest.mark.parametrize("dims", [(17, 8, 12), (9, 16, 6), (33, 4, 8)])
+def test_poisson_100_is_cosine_fourier_solve(dims):
+    """poisson_100 restated step by step (x reorder, r2c along z, shift
+    factors in z and y with the upper-half sign, x pairing, real K) equals
+    a direct solve in the cosine (x) by Fourier (y, z) basis: scipy DCT-II
+    along x, numpy FFTs along y and z, p_hat = -f_hat / K with K zero below
+    1e-16.  A planted sign on the y shift, or a dropped upper-half sign,
+    fails it."""
+    import scipy.fft as sf
+    p = O.Params(*dims, nclx1=1, nclxn=1)
+    S = O.Schemes(p)
+    f = np.random.default_rng(9).standard_normal((p.nz, p.ny, p.nxm))
+    a = O.poisson(S, f)
+    kx2, tx = O._wave_1d(S.st["x"], p.dx, p.nx, p.nxm, p.xlx, 1)
+    ky2, ty = O._wave_1d(S.st["y"], p.dy, p.ny, p.nym, p.yly, 0)
+    kz2, tz = O._wave_1d(S.st["z"], p.dz, p.nz, p.nzm, p.zlz, 0)
+    K = (kx2[None, None, :] * (ty[None, :, None] * tz[:, None, None]) ** 2
+         + ky2[None, :, None] * (tx[None, None, :] * tz[:, None, None]) ** 2
+         + kz2[:, None, None] * (tx[None, None, :] * ty[None, :, None]) ** 2)
+    F = np.fft.fftn(sf.dct(f, type=2, axis=2), axes=(0, 1))
+    Z = K < 1e-16
+    P = np.where(Z, 0.0, F / (-np.where(Z, 1.0, K)))
+    b = sf.idct(np.fft.ifftn(P, axes=(0, 1)).real, type=2, axis=2)
+    assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
+
+
+def test_poisson_100_inverts_freeslip_x_div_grad():
+    """Free-slip faces in x, periodic y and z: div(grad p) of the staggered
+    operators is diagonal in the cosine-Fourier basis, and the solve returns
+    p minus its mean.  The modes Nyquist in both y and z are in its null
+    space (each term of div(grad) carries the mid-point interpolation's
+    transfer function in y or z, zero at w = pi), so the field is drawn
+    without them."""
+    p = O.Params(17, 8, 12, nclx1=1, nclxn=1)
+    S = O.Schemes(p)
+    pe = np.random.default_rng(8).standard_normal((p.nz, p.ny, p.nxm))
+    F = np.fft.fftn(pe, axes=(0, 1))
+    F[p.nz // 2, p.ny // 2] = 0.0
+    pe = np.fft.ifftn(F, axes=(0, 1)).real
+    pr = O.poisson(S, O.divergence(S, *O.gradp(S, pe)))
+    assert np.abs(pr - (pe - pe.mean())).max() <= 1e-13 * np.abs(pe).max()
+
+
 # ------------------------------------------------------------------ stretched y (istret = 1, 2)
 @pytest.mark.parametrize("istret", [1, 2, 3])
 def test_stretching_metric_is_the_inverse_jacobian(istret):
