@@ -228,7 +228,7 @@ __device__ T boundary_row(int i, int n_in, int n_out, const Sten<T> &s, int bc1,
 //
 // LDS: cf and cb hold N*P*TL carries each, gf and gb 2*P segment factors,
 // x0s and xns N*TL line ends.
-template <typename T, int L, int N, int N1 = N>
+template <typename T, int L, int N, int N1 = N, bool PER = true>
 __device__ __forceinline__ void seg_solve(T (&v)[N][L], const int e0, const T *__restrict__ c1, const T alfa1,
                                           const T rden1, const T *__restrict__ c2, const T alfa2, const T rden2,
                                           T *cf, T *cb, T *gf, T *gb, T *x0s, T *xns, const int line,
@@ -345,7 +345,9 @@ __device__ __forceinline__ void seg_solve(T (&v)[N][L], const int e0, const T *_
       for (int q = 0; q < N; ++q) v[q][m] += (q < N1 ? hh1 : hh2) * c[q];
     }
   }
-  // periodic closure (Sherman-Morrison with the precomputed vector sm)
+  // periodic closure (Sherman-Morrison with the precomputed vector sm);
+  // a non-periodic line (PER false) is solved at this point
+  if constexpr (!PER) return;
 #pragma unroll
   for (int q = 0; q < N; ++q) {
     if (seg == 0) x0s[q * TL + line] = v[q][0];
@@ -890,6 +892,142 @@ static int launch_seg_line(const LineArgs<T> &a, T rden, const YzShape<T> &Y, bo
   return X3D_OK;
 }
 
+// ---------------------------------------------------------------- non-periodic y/z lines, segmented
+// One operator on y or z lines with free-slip or wall faces (the _11 _12 _21
+// _22 variants of derive.f90): the layout and solve of seg_line_kernel (TL
+// lines of 128-byte rows x P segments of L points, the set's truncated carry
+// chain), with the window clamped at the line ends, the boundary-closure
+// rows (at most four per end) evaluated once per line into LDS before the
+// barrier, and no periodic closure.  The last segment may be partial: the
+// coefficient table is padded in LDS up to P*L with decoupled rows (zero
+// multipliers and a zero right-hand side), which carry nothing across.
+// Input and output strides differ for the staggered operators (n_in !=
+// n_out); byte offsets are 32-bit (the host checks both field sizes).
+template <typename T, int OP, int L, bool EPI, int LB>
+__global__ void __launch_bounds__(LB) seg_wall_kernel(const LineArgs<T> A, const int kdepth) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *smem = reinterpret_cast<T *>(smem_raw);
+  const int TL = blockDim.x, P = blockDim.y;
+  const int line = threadIdx.x, seg = threadIdx.y;
+  const int tid = seg * TL + line, nthr = TL * P;
+  const int n_in = A.n_in, n_out = A.n_out, np = P * L;
+  T *cs = smem, *cf = cs + 4 * np, *cb = cf + P * TL, *gf = cb + P * TL, *gb = gf + P, *bnd = gb + P;
+  const int a = blockIdx.x * TL + line, b = blockIdx.y;
+  const bool active = a < A.na;
+  const int al = active ? a : A.na - 1;
+  const int e0 = seg * L;
+  const unsigned ibase = ((unsigned)al * (unsigned)A.sa_in + (unsigned)b * (unsigned)A.sb_in) * (unsigned)sizeof(T);
+  const unsigned istep = (unsigned)A.se_in * (unsigned)sizeof(T);
+  auto ld = [&](int j) -> T {
+    const unsigned o = ibase + (unsigned)j * istep;
+    T x = at(A.u, o);
+    if (EPI && A.u2) x *= at(A.u2, o);
+    return x;
+  };
+  // boundary rows 0..3 and n_out-4..n_out-1 of this line (the threads of
+  // the first eight segments), before the window is live
+  for (int q = seg; q < 8; q += P) {
+    const int r = q < 4 ? q : n_out - 8 + q;
+    if (is_boundary_row<OP>(r, n_out))
+      bnd[line * 8 + q] = boundary_row<OP, T>(r, n_in, n_out, A.s, A.bc1, A.bcn, A.npaire, ld);
+  }
+  T win[L + 8];
+#pragma unroll
+  for (int k = 0; k < L + 8; ++k) {
+    const int j = e0 - 4 + k;
+    win[k] = ld(j < 0 ? 0 : (j >= n_in ? n_in - 1 : j));
+  }
+  for (int i = tid; i < 4 * np; i += nthr) cs[i] = i < 4 * n_out ? A.pk[i] : T(0);
+  __syncthreads();
+  T w[1][L];
+#pragma unroll
+  for (int m = 0; m < L; ++m) {
+    const int e = e0 + m;
+    T r = T(0);
+    if (e < n_out)
+      r = is_boundary_row<OP>(e, n_out) ? bnd[line * 8 + (e < 4 ? e : e - (n_out - 8))]
+                                        : interior_row<OP, T, L + 8>(win, m + 4, A.s);
+    w[0][m] = r;
+  }
+  seg_solve<T, L, 1, 1, false>(w, e0, cs, A.alfa, T(0), cs, A.alfa, T(0), cf, cb, gf, gb, nullptr, nullptr, line,
+                               seg, TL, P, kdepth);
+  if (!active) return;
+  const unsigned obase =
+      ((unsigned)al * (unsigned)A.sa_out + (unsigned)b * (unsigned)A.sb_out) * (unsigned)sizeof(T);
+  const unsigned ostep = (unsigned)A.se_out * (unsigned)sizeof(T);
+  if (A.pp) {
+#pragma unroll
+    for (int m = 0; m < L; ++m)
+      if (e0 + m < n_out) w[0][m] *= A.pp[e0 + m];
+  }
+  if constexpr (EPI) {
+    T mm[L], aa[L];
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      const unsigned o = obase + (unsigned)(e0 + m) * ostep;
+      const bool in = e0 + m < n_out;
+      mm[m] = (A.mul && in) ? at(A.mul, o) : T(1);
+      aa[m] = (A.acc && in) ? at(A.acc, o) : T(0);
+    }
+#pragma unroll
+    for (int m = 0; m < L; ++m) {
+      if (e0 + m >= n_out) continue;
+      T r = w[0][m];
+      if (A.mul) r *= mm[m];
+      r = A.rc * r;
+      if (A.acc) r = A.ac * aa[m] + r;
+      at(A.t, obase + (unsigned)(e0 + m) * ostep) = r;
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < L; ++m)
+      if (e0 + m < n_out) at(A.t, obase + (unsigned)(e0 + m) * ostep) = w[0][m];
+  }
+}
+
+// Shape of seg_wall_kernel: 128-byte rows, segments of the precision's
+// segment length, doubled when the line needs more than 1024 threads (up
+// to 640 threads then, so that the longer windows keep a register budget of
+// 640 threads and stay out of scratch); longer lines go to line_kernel.
+template <typename T>
+struct WallShape {
+  static constexpr int L1 = sizeof(T) == 8 ? SEG_L64 : SEG_L32;
+  int L = 0, P = 0, TL = 0;
+  size_t shmem = 0;
+  bool plan(int n_out) {
+    if (n_out < 8) return false;
+    TL = 128 / (int)sizeof(T);
+    L = L1;
+    P = (n_out + L - 1) / L;
+    if (TL * P > 1024) {
+      TL = 128 / (int)sizeof(T);
+      L = 2 * L1;
+      P = (n_out + L - 1) / L;
+      if (TL * P > 640) return false;
+    }
+    shmem = sizeof(T) * ((size_t)4 * P * L + 2 * (size_t)P * TL + 2 * (size_t)P + 8 * (size_t)TL);
+    return shmem <= 65536;
+  }
+};
+
+template <typename T, int OP>
+static int launch_seg_wall(const LineArgs<T> &a, const WallShape<T> &Y, bool epi, int kdepth, hipStream_t st) {
+  const dim3 grid((a.na + Y.TL - 1) / Y.TL, a.nb), block(Y.TL, Y.P);
+  constexpr int L1 = WallShape<T>::L1;
+  // the set's chain depth counts segments of L1 points
+  const int K = std::min(Y.P, Y.L == L1 ? kdepth : (kdepth + 1) / 2);
+  if (Y.L == L1) {
+    if (epi) hipLaunchKernelGGL((seg_wall_kernel<T, OP, L1, true, 1024>), grid, block, Y.shmem, st, a, K);
+    else hipLaunchKernelGGL((seg_wall_kernel<T, OP, L1, false, 1024>), grid, block, Y.shmem, st, a, K);
+  } else {
+    if (epi) hipLaunchKernelGGL((seg_wall_kernel<T, OP, 2 * L1, true, 640>), grid, block, Y.shmem, st, a, K);
+    else hipLaunchKernelGGL((seg_wall_kernel<T, OP, 2 * L1, false, 640>), grid, block, Y.shmem, st, a, K);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(X3D_ERR_HIP, std::string("segmented wall-line launch: ") + hipGetErrorString(e));
+  return X3D_OK;
+}
+
 // ---------------------------------------------------------------- x lines, one wavefront per line
 // Periodic x lines of n = 64*V points: lane k owns points [kV, kV+V) of the
 // line (contiguous, so a wavefront reads/writes the line as one 64*V-word
@@ -1161,6 +1299,20 @@ int launch_line(const LineCall<T> &c, hipStream_t st) {
       case OP_INTERVP: return launch_seg_line<T, OP_INTERVP>(a, rden, Y, epi, K, st);
       case OP_DERPV: return launch_seg_line<T, OP_DERPV>(a, rden, Y, epi, K, st);
       case OP_INTERPV: return launch_seg_line<T, OP_INTERPV>(a, rden, Y, epi, K, st);
+    }
+  }
+  WallShape<T> Wsh;
+  const size_t fin = (size_t)nx * ny * nz / (size_t)(c.dir == 1 ? ny : nz) * (size_t)n_in * sizeof(T);
+  const size_t fout = (size_t)nx * ny * nz / (size_t)(c.dir == 1 ? ny : nz) * (size_t)n_out * sizeof(T);
+  if (!xmode && !periodic && Wsh.plan(n_out) && fin < ((size_t)1 << 32) && fout < ((size_t)1 << 32)) {
+    const int K = c.tri.kdepth > 0 ? c.tri.kdepth : Wsh.P;
+    switch (c.op) {
+      case OP_DER1: return launch_seg_wall<T, OP_DER1>(a, Wsh, epi, K, st);
+      case OP_DER2: return launch_seg_wall<T, OP_DER2>(a, Wsh, epi, K, st);
+      case OP_DERVP: return launch_seg_wall<T, OP_DERVP>(a, Wsh, epi, K, st);
+      case OP_INTERVP: return launch_seg_wall<T, OP_INTERVP>(a, Wsh, epi, K, st);
+      case OP_DERPV: return launch_seg_wall<T, OP_DERPV>(a, Wsh, epi, K, st);
+      case OP_INTERPV: return launch_seg_wall<T, OP_INTERPV>(a, Wsh, epi, K, st);
     }
   }
   const int ax = c.dir;
