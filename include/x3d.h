@@ -349,6 +349,14 @@ int x3d_solver_tgv_stats(x3d_solver *s, double out4[4]);
 /* Max and mean |div u| on the pressure grid after the projection
  * (navier.f90:346-361 with nlock=2 semantics, FP64). */
 int x3d_solver_divergence_stats(x3d_solver *s, double out2[2]);
+/* The statistics the reference's divergence computes on every call
+ * (navier.f90:348-361, nlock=1): max and mean |.| of div u* on the pressure
+ * grid, u* the velocity before the pressure correction of the last
+ * (sub-)step, in the compute precision of the step (tmax starting at -1609
+ * as there).  The step reduces them per block as it goes (in the z
+ * divergence sweep on the fused path); this call finishes the reduction.
+ * X3D_ERR_STATE before the first step. */
+int x3d_solver_divu_stats(x3d_solver *s, double out2[2]);
 int x3d_solver_itime(x3d_solver *s, int *itime);
 /* Channel initial condition of init_channel (Case-Channel.f90:96-128, iin=1,
  * idir_stream=1): u = init_noise um (2r-1) + 1 - y^2, v, w = init_noise um

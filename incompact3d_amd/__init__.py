@@ -230,6 +230,13 @@ class Solver:
         check(_lib.lib().x3d_solver_divergence_stats(self._h, out), "divergence_stats")
         return tuple(out)
 
+    def divu_stats(self):
+        """(max, mean |.|) of div u* before the last pressure correction, as
+        navier.f90:348-361 computes them on every call of divergence."""
+        out = (C.c_double * 2)()
+        check(_lib.lib().x3d_solver_divu_stats(self._h, out), "divu_stats")
+        return out[0], out[1]
+
     def momentum_yz(self, nrep: int = 1) -> float:
         """x3d_solver_momentum_yz: enqueue nrep times the six fused y/z
         momentum sweeps of one step on the library stream; returns the HBM

@@ -95,6 +95,7 @@ def _declare(lib):
     lib.x3d_solver_step.argtypes = [P, C.c_int]
     lib.x3d_solver_tgv_stats.argtypes = [P, C.POINTER(C.c_double)]
     lib.x3d_solver_divergence_stats.argtypes = [P, C.POINTER(C.c_double)]
+    lib.x3d_solver_divu_stats.argtypes = [P, C.POINTER(C.c_double)]
     lib.x3d_solver_itime.argtypes = [P, IP]
     lib.x3d_solver_momentum_yz.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
     lib.x3d_solver_init_channel.argtypes = [P, C.c_double, C.c_ulonglong]

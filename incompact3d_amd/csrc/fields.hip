@@ -366,6 +366,24 @@ __global__ void __launch_bounds__(EW_THREADS) finish_absmax(const double *__rest
     out[1] = r1[0] / (double)n;
   }
 }
+// the two stages apart: partials of a field (RED_BLOCKS pairs), and the
+// finish over nb pairs of any origin
+template <typename T>
+int launch_absmax_partials(const T *a, size_t n, double *partial, int *nb, hipStream_t st) {
+  hipLaunchKernelGGL((absmax_partials<T>), dim3(RED_BLOCKS), dim3(EW_THREADS), 0, st, a, n, partial);
+  X3D_HIP(hipGetLastError());
+  *nb = RED_BLOCKS;
+  return X3D_OK;
+}
+template int launch_absmax_partials<double>(const double *, size_t, double *, int *, hipStream_t);
+template int launch_absmax_partials<float>(const float *, size_t, double *, int *, hipStream_t);
+
+int launch_finish_absmax(const double *partial, int nb, size_t n, double *out2, hipStream_t st) {
+  hipLaunchKernelGGL(finish_absmax, dim3(1), dim3(EW_THREADS), 0, st, partial, nb, n, out2);
+  X3D_HIP(hipGetLastError());
+  return X3D_OK;
+}
+
 template <typename T>
 int launch_max_mean(const T *a, size_t n, double *partial, double *out2, hipStream_t st) {
   hipLaunchKernelGGL((absmax_partials<T>), dim3(RED_BLOCKS), dim3(EW_THREADS), 0, st, a, n, partial);

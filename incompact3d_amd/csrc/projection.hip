@@ -76,6 +76,7 @@ struct ProjArgs {
   double *u64[3];               // GRAD_X: the FP64 velocity, corrected in place
   float *u32[3];                // GRAD_X, FP32 bulk: its FP32 copy (else null)
   const float *du[3];           // GRAD_X, FP32 bulk: the AB3 increment (else null)
+  double *divp;                 // DIV_Z: per-block max and sum |div u*| (null: none)
 };
 
 // ---------------------------------------------------------------- y / z sweeps
@@ -146,6 +147,43 @@ __global__ void __launch_bounds__(1024) proj_yz_kernel(const ProjArgs<T> A) {
         at(A.out[0], o) = w[0][m];
         at(A.out[1], o) = w[N - 1][m];
         at(A.out[2], o) = w[1][m];
+      }
+    }
+  }
+  if constexpr (SW == SW_DIV_Z) {
+    // the divergence statistics of navier.f90:348-361 (max and sum of |.|
+    // over the pressure grid) as per-block partials, from the values stored
+    if (A.divp) {
+      double mx = -1.0e300, sm = 0.0;
+      if (active) {
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+          const double v = (double)(w[0][m] + w[N - 1][m]);
+          mx = v > mx ? v : mx;
+          sm += fabs(v);
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double om = __shfl_xor(mx, off), os = __shfl_xor(sm, off);
+        mx = om > mx ? om : mx;
+        sm += os;
+      }
+      __shared__ double rmx[16], rsm[16];
+      if ((tid & 63) == 0) {
+        rmx[tid >> 6] = mx;
+        rsm[tid >> 6] = sm;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double bm = rmx[0], bs = rsm[0];
+        for (int i = 1; i < (nthr + 63) / 64; ++i) {
+          bm = rmx[i] > bm ? rmx[i] : bm;
+          bs += rsm[i];
+        }
+        const unsigned nblk = gridDim.x * gridDim.y, blk = blockIdx.y * gridDim.x + blockIdx.x;
+        A.divp[blk] = bm;
+        A.divp[nblk + blk] = bs;
       }
     }
   }
@@ -284,7 +322,8 @@ static void launch_x(const ProjArgs<T> &A, int V, int nlines, hipStream_t st) {
 // by SW_GRAD_X only (u32 null for FP64 compute).
 template <typename T>
 int launch_projection(const Context &c, int sw, const T *const in[3], T *const out[3], double *const u64[3],
-                      float *const u32[3], const float *const du[3], int nx, int ny, int nz, hipStream_t st) {
+                      float *const u32[3], const float *const du[3], int nx, int ny, int nz, hipStream_t st,
+                      double *divp, int *divp_blocks) {
   if (sw < SW_DIV_X || sw > SW_GRAD_X) return fail(X3D_ERR_ARG, "projection: unknown sweep");
   const int dir = (sw == SW_DIV_X || sw == SW_GRAD_X) ? 0 : (sw == SW_DIV_Y || sw == SW_GRAD_Y) ? 1 : 2;
   const bool pv = sw >= SW_GRAD_Z;
@@ -329,6 +368,10 @@ int launch_projection(const Context &c, int sw, const T *const in[3], T *const o
     if ((size_t)nx * ny * nz * sizeof(T) >= ((size_t)1 << 32))
       return fail(X3D_ERR_UNSUPPORTED, "fused y/z sweep: field larger than 4 GiB");
     A.kdepth = std::max(tI.kdepth, tD.kdepth);
+    if (sw == SW_DIV_Z && divp) {
+      A.divp = divp;
+      if (divp_blocks) *divp_blocks = ((A.na + Y.TL - 1) / Y.TL) * A.nb;
+    }
     switch (sw) {
       case SW_DIV_Y: launch_yz<T, SW_DIV_Y>(A, Y, st); break;
       case SW_DIV_Z: launch_yz<T, SW_DIV_Z>(A, Y, st); break;
@@ -343,9 +386,9 @@ int launch_projection(const Context &c, int sw, const T *const in[3], T *const o
 
 template int launch_projection<double>(const Context &, int, const double *const[3], double *const[3],
                                        double *const[3], float *const[3], const float *const[3], int, int, int,
-                                       hipStream_t);
+                                       hipStream_t, double *, int *);
 template int launch_projection<float>(const Context &, int, const float *const[3], float *const[3],
                                       double *const[3], float *const[3], const float *const[3], int, int, int,
-                                      hipStream_t);
+                                      hipStream_t, double *, int *);
 
 }  // namespace x3d

@@ -49,6 +49,9 @@ int launch_tgv_sums(const T *ux, const T *uy, const T *uz, const T *const g[9], 
 int launch_zero_face(double *u, float *u32, int nx, int ny, int nz, int dir, int idx, hipStream_t st);
 template <typename T>
 int launch_max_mean(const T *a, size_t n, double *partial, double *out2, hipStream_t st);
+template <typename T>
+int launch_absmax_partials(const T *a, size_t n, double *partial, int *nb, hipStream_t st);
+int launch_finish_absmax(const double *partial, int nb, size_t n, double *out2, hipStream_t st);
 // channel.hip
 int launch_channel_init(double *ux, double *uy, double *uz, const double *yc, int nx, int ny, int nz, double noise,
                         unsigned long long seed, hipStream_t st);
@@ -81,6 +84,10 @@ struct x3d_solver {
   void *w[5] = {};             // work fields, compute precision; after a step
                                // w[0] holds its (last sub-step's) pressure
   bool have_pressure = false;  // w[0] holds a pressure (a step has run)
+  // statistics of div u* of the last (sub-)step (navier.f90:348-361): per-block
+  // max and sum of |.| pairs, divp_n of them, reduced when asked for
+  double *divp = nullptr;
+  int divp_n = 0;
   void *diag = nullptr;        // 12 fields for the TGV statistics
   double *w64[4] = {};         // FP64 work for the divergence check of precision 1
   double *partial = nullptr;   // reduction partials
@@ -412,6 +419,7 @@ int step_rk3(Context &c, x3d_solver *S) {
     std::swap(S->slot[0], S->slot[1]);   // f2 <- f1
     X3D_TRY(pre_correc(c, S, gdt[itr]));
     X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));
+    X3D_TRY(launch_absmax_partials<T>(W[0], (size_t)c.nxm * c.nym * c.nzm, S->divp, &S->divp_n, c.stream));
     X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
     X3D_TRY(gradp_correct<T>(c, S, u, W, gdt[itr]));
   }
@@ -503,7 +511,8 @@ int step_once(Context &c, x3d_solver *S) {
     X3D_TRY(launch_projection<T>(c, SW_DIV_Y, dy_in, dy_out, nullptr, nullptr, nullptr, nx, ny, nz, c.stream));
     const T *const dz_in[3] = {W[3], W[1], nullptr};
     T *const dz_out[3] = {W[0], nullptr, nullptr};
-    X3D_TRY(launch_projection<T>(c, SW_DIV_Z, dz_in, dz_out, nullptr, nullptr, nullptr, nx, ny, nz, c.stream));
+    X3D_TRY(launch_projection<T>(c, SW_DIV_Z, dz_in, dz_out, nullptr, nullptr, nullptr, nx, ny, nz, c.stream,
+                                 S->divp, &S->divp_n));
     X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
     const T *const gz_in[3] = {W[0], nullptr, nullptr};
     T *const gz_out[3] = {W[1], W[2], nullptr};
@@ -520,6 +529,7 @@ int step_once(Context &c, x3d_solver *S) {
   } else {
     X3D_TRY(pre_correc(c, S, gdt));
     X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));
+    X3D_TRY(launch_absmax_partials<T>(W[0], (size_t)c.nxm * c.nym * c.nzm, S->divp, &S->divp_n, c.stream));
     X3D_TRY(poisson_solve<T>(c, W[0], c.stream));
     X3D_TRY(gradp_correct<T>(c, S, u, W, gdt));
   }
@@ -585,6 +595,9 @@ int alloc_solver(Context &c, x3d_solver *S) {
   const size_t nw = n > np ? n : np;
   for (int i = 0; i < 5; ++i) X3D_HIP(hipMalloc(&S->w[i], nw * es));
   X3D_HIP(hipMalloc(&S->partial, 8 * 1024 * sizeof(double)));
+  // div u* partials: RED_BLOCKS pairs (per-operator step) or one pair per
+  // block of the fused z divergence sweep (at most nx * ny / 16 blocks)
+  X3D_HIP(hipMalloc(&S->divp, 2 * ((size_t)S->nx * S->ny + 1024) * sizeof(double)));
   X3D_HIP(hipMalloc(&S->red, 8 * sizeof(double)));
   if (S->wall1 || S->walln) X3D_HIP(hipMalloc(&S->wallg, 4 * (size_t)S->nx * S->nz * sizeof(double)));
   X3D_HIP(hipStreamSynchronize(c.stream));
@@ -608,6 +621,7 @@ void free_solver(x3d_solver *S) {
   if (S->lesw) (void)hipFree(S->lesw);
   if (S->nfb) (void)hipFree(S->nfb);
   if (S->partial) (void)hipFree(S->partial);
+  if (S->divp) (void)hipFree(S->divp);
   if (S->red) (void)hipFree(S->red);
   if (S->wallg) (void)hipFree(S->wallg);
   if (S->ycoord) (void)hipFree(S->ycoord);
@@ -624,6 +638,7 @@ int reset_history(Context &c, x3d_solver *S) {
   S->slot[2] = 2;
   S->itime = 0;
   S->have_pressure = false;
+  S->divp_n = 0;
   // dpdxy1 .. dpdzyn start at zero (the wall velocity of the first pre_correc)
   if (S->wallg) X3D_HIP(hipMemsetAsync(S->wallg, 0, 4 * (size_t)S->nx * S->nz * sizeof(double), c.stream));
   if (S->precision == 1)
@@ -754,6 +769,20 @@ int x3d_solver_divergence_stats(x3d_solver *S, double out2[2]) {
   X3D_TRY(launch_max_mean<double>(W[0], np, S->partial, S->red, c->stream));
   X3D_HIP(hipMemcpyAsync(out2, S->red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   X3D_HIP(hipStreamSynchronize(c->stream));
+  return X3D_OK;
+}
+
+int x3d_solver_divu_stats(x3d_solver *S, double out2[2]) {
+  Context *c;
+  X3D_TRY(require_ctx(&c));
+  if (!S || !out2) return fail(X3D_ERR_ARG, "null pointer");
+  if (S->divp_n == 0) return fail(X3D_ERR_STATE, "div u* statistics: no step has run since the field was set");
+  const size_t np = (size_t)c->nxm * c->nym * c->nzm;
+  X3D_TRY(launch_finish_absmax(S->divp, S->divp_n, np, S->red, c->stream));
+  X3D_HIP(hipMemcpyAsync(out2, S->red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  X3D_HIP(hipStreamSynchronize(c->stream));
+  // tmax starts at -1609 in the reference
+  if (out2[0] < -1609.0) out2[0] = -1609.0;
   return X3D_OK;
 }
 

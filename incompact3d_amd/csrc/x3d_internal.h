@@ -207,7 +207,8 @@ enum { SW_DIV_X = 0, SW_DIV_Y, SW_DIV_Z, SW_GRAD_Z, SW_GRAD_Y, SW_GRAD_X };
 bool projection_fusable(const Context &c);
 template <typename T>
 int launch_projection(const Context &c, int sw, const T *const in[3], T *const out[3], double *const u64[3],
-                      float *const u32[3], const float *const du[3], int nx, int ny, int nz, hipStream_t st);
+                      float *const u32[3], const float *const du[3], int nx, int ny, int nz, hipStream_t st,
+                      double *divp = nullptr, int *divp_blocks = nullptr);
 
 // ---------------------------------------------------------------- closure network (mlp.hip)
 struct MlpDev;

@@ -1645,12 +1645,20 @@ class State:
     hy: list = field(default_factory=list)
     hz: list = field(default_factory=list)
     itime: int = 0
+    divu: tuple = None    # (max, mean |.|) of div u* of the last (sub-)step
 
 
 def tgv_state(p: Params):
     ux, uy, uz = init_tgv(p)
     # case.f90:131-136: RHS history seeded with u
     return State(ux, uy, uz, [ux.copy()] * 3, [uy.copy()] * 3, [uz.copy()] * 3, 0)
+
+
+def divu_stats(pp3):
+    """The statistics divergence computes on every call (navier.f90:348-361):
+    the maximum (starting from -1609) and the mean |.| of div u* over the
+    pressure grid."""
+    return max(-1609.0, float(pp3.max())), float(np.abs(pp3).mean())
 
 
 def step_rk3(S: Schemes, st: State):
@@ -1666,7 +1674,9 @@ def step_rk3(S: Schemes, st: State):
         st.uy = intt_rk3(st.uy, st.hy, itr, dt)
         st.uz = intt_rk3(st.uz, st.hz, itr, dt)
         st.ux, st.uy, st.uz = pre_correc(S.p, st.ux, st.uy, st.uz)
-        pp3 = poisson(S, divergence(S, st.ux, st.uy, st.uz))
+        pp3 = divergence(S, st.ux, st.uy, st.uz)
+        st.divu = divu_stats(pp3)
+        pp3 = poisson(S, pp3)
         px, py, pz = gradp(S, pp3)
         st.ux = st.ux - px
         st.uy = st.uy - py
@@ -1684,6 +1694,7 @@ def step(S: Schemes, st: State):
     st.uy = intt_ab3(st.uy, st.hy, st.itime, dt)
     st.uz = intt_ab3(st.uz, st.hz, st.itime, dt)
     pp3 = divergence(S, st.ux, st.uy, st.uz)
+    st.divu = divu_stats(pp3)
     pp3 = poisson(S, pp3)
     px, py, pz = gradp(S, pp3)
     st.ux = st.ux - px
@@ -1839,7 +1850,9 @@ def step_channel(S: Schemes, st: State, wg: WallGrad, cfr=2.0 / 3.0, fcpg=None, 
             st.uy = intt_ab3(st.uy, st.hy, st.itime, dt)
             st.uz = intt_ab3(st.uz, st.hz, st.itime, dt)
         st.ux, st.uy, st.uz = pre_correc_walls(p, st.ux, st.uy, st.uz, wg, gdt)
-        pp3 = poisson(S, divergence(S, st.ux, st.uy, st.uz))
+        pp3 = divergence(S, st.ux, st.uy, st.uz)
+        st.divu = divu_stats(pp3)
+        pp3 = poisson(S, pp3)
         px, py, pz = gradp(S, pp3)
         wg = gradp_walls(p, px, pz, wg, gdt)
         st.ux = st.ux - px
