@@ -224,6 +224,27 @@ def microbench(X, n, walls, stream, reps=10):
     return {"grid": [n, ny, n], "y": "Dirichlet walls (ncl=2)" if walls else "periodic", "ops": out}
 
 
+def c1_bench(X, stream, steps=200):
+    """The 64^3 TGV of the CPU-runnable configuration (Re 1600, AB3, dt 0.005,
+    200 steps): HIP-event ms per step over the whole run, FP64 and FP32 bulk
+    (each kernel is a few microseconds at this size, so the step is bound by
+    the ~20 launches, not by HBM)."""
+    p = X.make_params(64, 64, 64, re=1600.0, dt=0.005)
+    out = {"workload": "TGV 64^3 Re=1600, AB3 dt=0.005, 200 steps"}
+    X.init(p)
+    X.set_stream(stream.cuda_stream)
+    try:
+        for prec, lab in ((0, "fp64"), (1, "fp32_bulk")):
+            s = X.Solver(p, prec)
+            s.init_tgv()
+            ms = _events_time(stream, lambda: s.step(steps), 1) / steps
+            s.close()
+            out[lab] = {"ms_per_step": round(ms, 4), "mpoint_steps_per_s": round(64 ** 3 / (ms * 1e-3) / 1e6, 1)}
+    finally:
+        X.finalize()
+    return out
+
+
 def channel_step_bench(X, stream, warmup=20, steps=200):
     """The Re_tau 180 channel step (256 x 129 x 128, stretched y, RK3,
     hyperviscous second derivative, constant flow rate, spin-up rotation on)
@@ -400,7 +421,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the full-length (t=20) FP64 / FP32-bulk accuracy run (about 40 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-c4", action="store_true", help="skip the channel step timing (a few seconds)")
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the channel and 64^3 TGV step timings (a few seconds)")
     ap.add_argument("--no-les", action="store_true",
                     help="skip the learned-LES time-to-solution comparison (TGV Re=5000, about 40 s)")
     ap.add_argument("--micro-n", type=int, default=512,
@@ -494,6 +516,7 @@ def main():
         out["stretched_poisson"] = stretched_poisson_bench(X, stream)
     if not args.no_c4:
         out["channel_c4"] = channel_step_bench(X, stream)
+        out["tgv64_c1"] = c1_bench(X, stream)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
         out["cpu_baseline"]["host"] = f"{os.cpu_count()} logical CPUs visible"
