@@ -2,6 +2,7 @@
 // Taylor-Green initial condition and deterministic FP64 reductions.
 #include <cmath>
 
+#include "state64.h"
 #include "x3d_internal.h"
 
 namespace x3d {
@@ -152,6 +153,33 @@ __global__ void __launch_bounds__(EW_THREADS) cast_kernel(const double *__restri
 }
 int launch_cast(const double *a, float *b, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(cast_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, st, a, b, n);
+  X3D_HIP(hipGetLastError());
+  return X3D_OK;
+}
+// FP64 field <-> (FP32 rounding, integer residual) pair, exact both ways
+// for finite values of magnitude at least 2^-127 (state64.h)
+__global__ void __launch_bounds__(EW_THREADS) split_kernel(const double *__restrict__ a, float *__restrict__ hi,
+                                                           int32_t *__restrict__ r, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double u = a[i];
+    const float h = (float)u;
+    hi[i] = h;
+    r[i] = split_residual(u, h);
+  }
+}
+int launch_split(const double *a, float *hi, int32_t *r, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(split_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, st, a, hi, r, n);
+  X3D_HIP(hipGetLastError());
+  return X3D_OK;
+}
+__global__ void __launch_bounds__(EW_THREADS) join_kernel(const float *__restrict__ hi, const int32_t *__restrict__ r,
+                                                          double *__restrict__ a, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) a[i] = join_split(hi[i], r[i]);
+}
+int launch_join(const float *hi, const int32_t *r, double *a, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(join_kernel, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, st, hi, r, a, n);
   X3D_HIP(hipGetLastError());
   return X3D_OK;
 }

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "lines_impl.h"
+#include "state64.h"
 
 namespace x3d {
 
@@ -76,6 +77,12 @@ struct ProjArgs {
   double *u64[3];               // GRAD_X: the FP64 velocity, corrected in place
   float *u32[3];                // GRAD_X, FP32 bulk: its FP32 copy (else null)
   const float *du[3];           // GRAD_X, FP32 bulk: the AB3 increment (else null)
+  // GRAD_X, FP32 bulk with the FP64 velocity held as (FP32 rounding, integer
+  // residual) pairs (state64.h): hold is the rounding before the step, res
+  // the residual (read, then rewritten), u32 receives the new rounding; u64
+  // is then unused (null)
+  const float *hold[3];
+  int32_t *res[3];
   double *divp;                 // DIV_Z: per-block max and sum |div u*| (null: none)
 };
 
@@ -192,11 +199,22 @@ __global__ void __launch_bounds__(1024) proj_yz_kernel(const ProjArgs<T> A) {
 // ---------------------------------------------------------------- x sweeps (one wavefront per line)
 // cor_vel (navier.f90:206-249) on one run of V points: u -= g in FP64, and
 // for FP32-bulk compute first u += the step's AB3 increment, then the FP32
-// copy refreshed
+// copy refreshed.  The FP64 velocity is read from and written to either a
+// plain FP64 array or its split form (hold/res in, u32/res out).
 template <int V, typename T>
 __device__ __forceinline__ void correct(const ProjArgs<T> &A, const int c, const long long base, const T (&g)[V]) {
   double u[V];
-  load_run<double, V>(A.u64[c] + base, u);
+  const bool split = sizeof(T) == 4 && A.res[c];
+  if (split) {
+    float h[V];
+    int32_t r[V];
+    load_run<float, V>(A.hold[c] + base, h);
+    load_run<int32_t, V>(A.res[c] + base, r);
+#pragma unroll
+    for (int m = 0; m < V; ++m) u[m] = join_split(h[m], r[m]);
+  } else {
+    load_run<double, V>(A.u64[c] + base, u);
+  }
   if (sizeof(T) == 4 && A.du[c]) {
     float d[V];
     load_run<float, V>(A.du[c] + base, d);
@@ -205,6 +223,18 @@ __device__ __forceinline__ void correct(const ProjArgs<T> &A, const int c, const
   }
 #pragma unroll
   for (int m = 0; m < V; ++m) u[m] = u[m] - (double)g[m];
+  if (split) {
+    float f[V];
+    int32_t r[V];
+#pragma unroll
+    for (int m = 0; m < V; ++m) {
+      f[m] = (float)u[m];
+      r[m] = split_residual(u[m], f[m]);
+    }
+    store_run<float, V>(A.u32[c] + base, f);
+    store_run<int32_t, V>(A.res[c] + base, r);
+    return;
+  }
   store_run<double, V>(A.u64[c] + base, u);
   if (sizeof(T) == 4 && A.u32[c]) {
     float f[V];
@@ -323,7 +353,7 @@ static void launch_x(const ProjArgs<T> &A, int V, int nlines, hipStream_t st) {
 template <typename T>
 int launch_projection(const Context &c, int sw, const T *const in[3], T *const out[3], double *const u64[3],
                       float *const u32[3], const float *const du[3], int nx, int ny, int nz, hipStream_t st,
-                      double *divp, int *divp_blocks) {
+                      double *divp, int *divp_blocks, const float *const hold[3], int32_t *const res[3]) {
   if (sw < SW_DIV_X || sw > SW_GRAD_X) return fail(X3D_ERR_ARG, "projection: unknown sweep");
   const int dir = (sw == SW_DIV_X || sw == SW_GRAD_X) ? 0 : (sw == SW_DIV_Y || sw == SW_GRAD_Y) ? 1 : 2;
   const bool pv = sw >= SW_GRAD_Z;
@@ -338,6 +368,8 @@ int launch_projection(const Context &c, int sw, const T *const in[3], T *const o
     A.u64[i] = u64 ? u64[i] : nullptr;
     A.u32[i] = u32 ? u32[i] : nullptr;
     A.du[i] = du ? du[i] : nullptr;
+    A.hold[i] = hold ? hold[i] : nullptr;
+    A.res[i] = res ? res[i] : nullptr;
   }
   TriDev<T> tI, tD;
   if constexpr (sizeof(T) == 8) { tI = sI.dev64(); tD = sD.dev64(); }
@@ -352,8 +384,14 @@ int launch_projection(const Context &c, int sw, const T *const in[3], T *const o
     const int V = n / 64;
     if (n % 64 != 0 || !(V == 1 || V == 2 || V == 4 || V == 8))
       return fail(X3D_ERR_UNSUPPORTED, "fused x sweep needs nx = 64, 128, 256 or 512");
-    if (sw == SW_GRAD_X && (!u64 || !u64[0] || !u64[1] || !u64[2]))
-      return fail(X3D_ERR_ARG, "projection: GRAD_X needs the FP64 velocity");
+    if (sw == SW_GRAD_X) {
+      bool ok = true;
+      for (int i = 0; i < 3; ++i) {
+        if (res && sizeof(T) == 4) ok = ok && hold && hold[i] && res[i] && u32 && u32[i] && du && du[i];
+        else ok = ok && u64 && u64[i];
+      }
+      if (!ok) return fail(X3D_ERR_ARG, "projection: GRAD_X needs the FP64 velocity (plain or split)");
+    }
     const int nlines = ny * nz;
     if (sw == SW_DIV_X) launch_x<T, SW_DIV_X>(A, V, nlines, st);
     else launch_x<T, SW_GRAD_X>(A, V, nlines, st);
@@ -386,9 +424,9 @@ int launch_projection(const Context &c, int sw, const T *const in[3], T *const o
 
 template int launch_projection<double>(const Context &, int, const double *const[3], double *const[3],
                                        double *const[3], float *const[3], const float *const[3], int, int, int,
-                                       hipStream_t, double *, int *);
+                                       hipStream_t, double *, int *, const float *const[3], int32_t *const[3]);
 template int launch_projection<float>(const Context &, int, const float *const[3], float *const[3],
                                       double *const[3], float *const[3], const float *const[3], int, int, int,
-                                      hipStream_t, double *, int *);
+                                      hipStream_t, double *, int *, const float *const[3], int32_t *const[3]);
 
 }  // namespace x3d

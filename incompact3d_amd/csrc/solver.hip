@@ -40,6 +40,8 @@ template <typename P>
 int launch_corvel(double *u, const P *p, float *shadow, size_t n, hipStream_t st);
 int launch_cast(const double *a, float *b, size_t n, hipStream_t st);
 int launch_widen(const float *a, double *b, size_t n, hipStream_t st);
+int launch_split(const double *a, float *hi, int32_t *r, size_t n, hipStream_t st);
+int launch_join(const float *hi, const int32_t *r, double *a, size_t n, hipStream_t st);
 int launch_tgv(double *ux, double *uy, double *uz, int nx, int ny, int nz, double dx, double dy, double dz,
                hipStream_t st);
 template <typename T>
@@ -79,6 +81,12 @@ struct x3d_solver {
   float *u32[3] = {nullptr, nullptr, nullptr};    // its FP32 copy (precision 1)
   double *u64b[3] = {nullptr, nullptr, nullptr};  // next-step buffers (AB3 writes here,
   float *u32b[3] = {nullptr, nullptr, nullptr};   // then the two sets swap)
+  // fused FP32-bulk step: the FP64 velocity lives between steps as u32 (its
+  // FP32 rounding) plus res (an integer residual), exact (state64.h); u64 is
+  // rebuilt from the pair when a caller asks for it (u64_stale)
+  bool split = false;
+  bool u64_stale = false;
+  int32_t *res[3] = {nullptr, nullptr, nullptr};
   void *hist[3][3] = {};       // [component][slot], compute precision
   int slot[3] = {0, 1, 2};     // slot index of f1, f2, f3
   void *w[5] = {};             // work fields, compute precision; after a step
@@ -524,8 +532,16 @@ int step_once(Context &c, x3d_solver *S) {
     double *const u64[3] = {S->u64[0], S->u64[1], S->u64[2]};
     float *const u32[3] = {S->u32[0], S->u32[1], S->u32[2]};
     const float *const du[3] = {(const float *)S->u64b[0], (const float *)S->u64b[1], (const float *)S->u64b[2]};
-    X3D_TRY(launch_projection<T>(c, SW_GRAD_X, gx_in, onone, u64, sizeof(T) == 8 ? nullptr : u32,
-                                 sizeof(T) == 8 ? nullptr : du, nx, ny, nz, c.stream));
+    if (S->split) {
+      // after the swap u32b holds the rounding of the velocity before the step
+      const float *const hold[3] = {S->u32b[0], S->u32b[1], S->u32b[2]};
+      X3D_TRY(launch_projection<T>(c, SW_GRAD_X, gx_in, onone, nullptr, u32, du, nx, ny, nz, c.stream, nullptr,
+                                   nullptr, hold, S->res));
+      S->u64_stale = true;
+    } else {
+      X3D_TRY(launch_projection<T>(c, SW_GRAD_X, gx_in, onone, u64, sizeof(T) == 8 ? nullptr : u32,
+                                   sizeof(T) == 8 ? nullptr : du, nx, ny, nz, c.stream));
+    }
   } else {
     X3D_TRY(pre_correc(c, S, gdt));
     X3D_TRY(divergence<T>(c, cu, W, nx, ny, nz));
@@ -586,6 +602,7 @@ int alloc_solver(Context &c, x3d_solver *S) {
       X3D_HIP(hipMalloc(&S->u64b[i], n * 8));
       if (S->precision == 1) X3D_HIP(hipMalloc(&S->u32b[i], n * 4));
     }
+    if (S->split) X3D_HIP(hipMalloc(&S->res[i], n * 4));
     for (int j = 0; j < 3; ++j) {
       X3D_HIP(hipMalloc(&S->hist[i][j], n * es));
       X3D_HIP(hipMemsetAsync(S->hist[i][j], 0, n * es, c.stream));
@@ -610,6 +627,7 @@ void free_solver(x3d_solver *S) {
     if (S->u32[i]) (void)hipFree(S->u32[i]);
     if (S->u64b[i]) (void)hipFree(S->u64b[i]);
     if (S->u32b[i]) (void)hipFree(S->u32b[i]);
+    if (S->res[i]) (void)hipFree(S->res[i]);
     for (int j = 0; j < 3; ++j)
       if (S->hist[i][j]) (void)hipFree(S->hist[i][j]);
   }
@@ -627,6 +645,25 @@ void free_solver(x3d_solver *S) {
   if (S->ycoord) (void)hipFree(S->ycoord);
 }
 
+// u64 was just written: refresh its FP32 copy (and, in split form, the
+// residual)
+int narrow_velocity(Context &c, x3d_solver *S) {
+  for (int i = 0; i < 3; ++i) {
+    if (S->split) X3D_TRY(launch_split(S->u64[i], S->u32[i], S->res[i], S->n, c.stream));
+    else if (S->precision == 1) X3D_TRY(launch_cast(S->u64[i], S->u32[i], S->n, c.stream));
+  }
+  S->u64_stale = false;
+  return X3D_OK;
+}
+
+// u64 is about to be read: rebuild it from the split form if a step changed it
+int sync_velocity(Context &c, x3d_solver *S) {
+  if (!S->split || !S->u64_stale) return X3D_OK;
+  for (int i = 0; i < 3; ++i) X3D_TRY(launch_join(S->u32[i], S->res[i], S->u64[i], S->n, c.stream));
+  S->u64_stale = false;
+  return X3D_OK;
+}
+
 int reset_history(Context &c, x3d_solver *S) {
   // case.f90:131-136 seeds the history with u; step 1 (Euler) and step 2
   // (AB2) never read the seeded values, so the slots only need defined data.
@@ -641,8 +678,7 @@ int reset_history(Context &c, x3d_solver *S) {
   S->divp_n = 0;
   // dpdxy1 .. dpdzyn start at zero (the wall velocity of the first pre_correc)
   if (S->wallg) X3D_HIP(hipMemsetAsync(S->wallg, 0, 4 * (size_t)S->nx * S->nz * sizeof(double), c.stream));
-  if (S->precision == 1)
-    for (int i = 0; i < 3; ++i) X3D_TRY(launch_cast(S->u64[i], S->u32[i], S->n, c.stream));
+  X3D_TRY(narrow_velocity(c, S));
   X3D_HIP(hipStreamSynchronize(c.stream));
   return X3D_OK;
 }
@@ -680,6 +716,7 @@ int x3d_solver_create(int precision, x3d_solver **out) {
   S->walln = c->dir[1].bcn == 2;
   S->fused = c->p.itimescheme == 3 && c->p.istret == 0 && momentum_fusable(*c) && projection_fusable(*c) &&
              getenv("X3D_UNFUSED_MOMENTUM") == nullptr;
+  S->split = S->fused && precision == 1;
   int r = alloc_solver(*c, S);
   if (r != X3D_OK) {
     free_solver(S);
@@ -722,6 +759,7 @@ int x3d_solver_get_velocity(x3d_solver *S, double *ux, double *uy, double *uz) {
   X3D_TRY(require_ctx(&c));
   if (!S || !ux || !uy || !uz) return fail(X3D_ERR_ARG, "null pointer");
   double *dst[3] = {ux, uy, uz};
+  X3D_TRY(sync_velocity(*c, S));
   for (int i = 0; i < 3; ++i)
     X3D_HIP(hipMemcpyAsync(dst[i], S->u64[i], S->n * 8, hipMemcpyDeviceToDevice, c->stream));
   X3D_HIP(hipStreamSynchronize(c->stream));
@@ -764,6 +802,7 @@ int x3d_solver_divergence_stats(x3d_solver *S, double out2[2]) {
   for (int i = 0; i < 4; ++i)
     if (!S->w64[i]) X3D_HIP(hipMalloc(&S->w64[i], nw * 8));
   double *W[4] = {S->w64[0], S->w64[1], S->w64[2], S->w64[3]};
+  X3D_TRY(sync_velocity(*c, S));
   const double *u[3] = {S->u64[0], S->u64[1], S->u64[2]};
   X3D_TRY(divergence<double>(*c, u, W, S->nx, S->ny, S->nz));
   X3D_TRY(launch_max_mean<double>(W[0], np, S->partial, S->red, c->stream));
@@ -905,6 +944,7 @@ int x3d_solver_bulk_velocity(x3d_solver *S, double *ub) {
   X3D_TRY(require_ctx(&c));
   if (!S || !ub) return fail(X3D_ERR_ARG, "null pointer");
   const double coeff = c->dir[1].d / (c->p.yly * (double)((long long)S->nx * S->nz));
+  X3D_TRY(sync_velocity(*c, S));
   X3D_TRY(launch_bulk(S->u64[0], c->p.istret != 0 ? c->ppy64 : nullptr, S->nx, S->ny, S->nz, coeff, 0.0, S->partial,
                       S->red, c->stream));
   X3D_HIP(hipMemcpyAsync(ub, S->red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1000,6 +1040,7 @@ int x3d_solver_checkpoint(x3d_solver *S, const char *rundir, double t) {
     return fail(X3D_ERR_IO, "checkpoint: cannot create " + dir + ": " + std::strerror(errno));
   const size_t es = S->precision == 0 ? 8 : 4;
   std::vector<double> h;
+  X3D_TRY(sync_velocity(*c, S));
   X3D_HIP(hipStreamSynchronize(c->stream));
   for (int i = 0; i < 3; ++i) {
     X3D_TRY(field_to_host(S->u64[i], 8, S->n, h, c->stream));
@@ -1068,8 +1109,8 @@ int x3d_solver_restart(x3d_solver *S, const char *rundir, double *t) {
   for (int i = 0; i < 3; ++i) {
     X3D_TRY(read_raw(dir + "/" + VNAME[i], h.data(), S->n));
     X3D_TRY(field_from_host(S->u64[i], 8, h, c->stream));
-    if (S->precision == 1) X3D_TRY(launch_cast(S->u64[i], S->u32[i], S->n, c->stream));
   }
+  X3D_TRY(narrow_velocity(*c, S));
   S->slot[0] = 0;
   S->slot[1] = 1;
   S->slot[2] = 2;

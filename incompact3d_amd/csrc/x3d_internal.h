@@ -208,7 +208,8 @@ bool projection_fusable(const Context &c);
 template <typename T>
 int launch_projection(const Context &c, int sw, const T *const in[3], T *const out[3], double *const u64[3],
                       float *const u32[3], const float *const du[3], int nx, int ny, int nz, hipStream_t st,
-                      double *divp = nullptr, int *divp_blocks = nullptr);
+                      double *divp = nullptr, int *divp_blocks = nullptr, const float *const hold[3] = nullptr,
+                      int32_t *const res[3] = nullptr);
 
 // ---------------------------------------------------------------- closure network (mlp.hip)
 struct MlpDev;
