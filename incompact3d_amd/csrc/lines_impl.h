@@ -559,6 +559,25 @@ __device__ __forceinline__ void wave_solve(T (&v)[V], const T (&fs)[V], const T 
   for (int m = 0; m < V; ++m) v[m] = v[m] - sx * sm[m];
 }
 
+// Block coordinates (line tile, plane) of a y/z sweep.  Workgroups are handed
+// to the eight XCDs in turn (id mod 8), so with a grid eight tiles wide each
+// XCD would read one 128-byte column of every row.  With XCD set (and a
+// block count divisible by 8) the ids are renumbered so that each XCD works
+// through one contiguous eighth of the (tile, plane) space: whole rows of
+// the field pass through one L2.
+template <bool XCD>
+__device__ __forceinline__ void block_coords(int &bx, int &by) {
+  const int gx = (int)gridDim.x, nblk = gx * (int)gridDim.y;
+  if (!XCD || (nblk & 7) != 0) {
+    bx = (int)blockIdx.x;
+    by = (int)blockIdx.y;
+    return;
+  }
+  int l = (int)blockIdx.x + gx * (int)blockIdx.y;
+  l = (l & 7) * (nblk >> 3) + (l >> 3);
+  by = l / gx;
+  bx = l - by * gx;
+}
 
 // Launch shape of a segmented y/z sweep for a line of n points: segments of
 // L = 8 (FP64) or 16 (FP32) points, so that the fused sweeps' three solves

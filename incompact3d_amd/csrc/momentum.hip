@@ -60,7 +60,9 @@ __global__ void __launch_bounds__(1024) mom_yz_kernel(const MomArgs<T> A) {
   // LDS: two coefficient tables, carries of three right-hand sides
   T *cs1 = smem, *cs2 = cs1 + 4 * n, *cf = cs2 + 4 * n, *cb = cf + 3 * P * TL;
   T *gf = cb + 3 * P * TL, *gb = gf + 2 * P, *x0s = gb + 2 * P, *xns = x0s + 3 * TL;
-  const int a = blockIdx.x * TL + line, b = blockIdx.y;
+  int bx, b;
+  block_coords<sizeof(T) == 4>(bx, b);
+  const int a = bx * TL + line;
   const bool active = a < A.na;
   // lines past the end compute on a copy of the last line (no load branches)
   const int al = active ? a : A.na - 1;

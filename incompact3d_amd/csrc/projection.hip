@@ -101,7 +101,9 @@ __global__ void __launch_bounds__(1024) proj_yz_kernel(const ProjArgs<T> A) {
   constexpr int N = R::NI + 1;   // interpolations, then the derivative
   T *csI = smem, *csD = csI + 4 * n, *cf = csD + 4 * n, *cb = cf + N * P * TL;
   T *gf = cb + N * P * TL, *gb = gf + 2 * P, *x0s = gb + 2 * P, *xns = x0s + N * TL;
-  const int a = blockIdx.x * TL + line, b = blockIdx.y;
+  int bx, b;
+  block_coords<sizeof(T) == 4>(bx, b);
+  const int a = bx * TL + line;
   const bool active = a < A.na;
   // lines past the end compute on a copy of the last line (no load branches)
   const int al = active ? a : A.na - 1;
