@@ -414,6 +414,15 @@ __device__ __forceinline__ const T &at(const T *p, unsigned byte_off) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + byte_off);
 }
 
+// Non-temporal (streaming) store at a byte offset, for output fields that
+// the kernel does not read again; which kernels use it was settled by
+// measurement (the segmented y/z derivative and gradient sweeps gained,
+// the momentum and divergence sweeps and the x lines lost).
+template <typename T>
+__device__ __forceinline__ void st_nt(T *p, unsigned byte_off, T v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<T *>(reinterpret_cast<char *>(p) + byte_off));
+}
+
 // ---------------------------------------------------------------- x wavefront solve
 // floor(a / b) for b > 0 (compile-time after unrolling)
 constexpr int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
@@ -889,11 +898,11 @@ __global__ void __launch_bounds__(1024) seg_line_kernel(const LineArgs<T> A, con
       if (A.mul) r *= mm[m];
       r = A.rc * r;
       if (A.acc) r = A.ac * aa[m] + r;
-      at(A.t, base + (unsigned)(e0 + m) * step) = r;
+      st_nt(A.t, base + (unsigned)(e0 + m) * step, r);
     }
   } else {
 #pragma unroll
-    for (int m = 0; m < L; ++m) at(A.t, base + (unsigned)(e0 + m) * step) = w[0][m];
+    for (int m = 0; m < L; ++m) st_nt(A.t, base + (unsigned)(e0 + m) * step, w[0][m]);
   }
 }
 
@@ -995,12 +1004,12 @@ __global__ void __launch_bounds__(LB) seg_wall_kernel(const LineArgs<T> A, const
       if (A.mul) r *= mm[m];
       r = A.rc * r;
       if (A.acc) r = A.ac * aa[m] + r;
-      at(A.t, obase + (unsigned)(e0 + m) * ostep) = r;
+      st_nt(A.t, obase + (unsigned)(e0 + m) * ostep, r);
     }
   } else {
 #pragma unroll
     for (int m = 0; m < L; ++m)
-      if (e0 + m < n_out) at(A.t, obase + (unsigned)(e0 + m) * ostep) = w[0][m];
+      if (e0 + m < n_out) st_nt(A.t, obase + (unsigned)(e0 + m) * ostep, w[0][m]);
   }
 }
 

@@ -149,13 +149,13 @@ __global__ void __launch_bounds__(1024) proj_yz_kernel(const ProjArgs<T> A) {
         at(A.out[1], o) = w[1][m];
       } else if constexpr (SW == SW_DIV_Z) {
         at(A.out[0], o) = w[0][m] + w[N - 1][m];
-      } else if constexpr (SW == SW_GRAD_Z) {
-        at(A.out[0], o) = w[0][m];
-        at(A.out[1], o) = w[N - 1][m];
-      } else {   // SW_GRAD_Y
-        at(A.out[0], o) = w[0][m];
-        at(A.out[1], o) = w[N - 1][m];
-        at(A.out[2], o) = w[1][m];
+      } else if constexpr (SW == SW_GRAD_Z) {   // streaming stores (see st_nt)
+        st_nt(A.out[0], o, w[0][m]);
+        st_nt(A.out[1], o, w[N - 1][m]);
+      } else {   // SW_GRAD_Y, streaming stores
+        st_nt(A.out[0], o, w[0][m]);
+        st_nt(A.out[1], o, w[N - 1][m]);
+        st_nt(A.out[2], o, w[1][m]);
       }
     }
   }
