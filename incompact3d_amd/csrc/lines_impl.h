@@ -931,8 +931,8 @@ static int launch_seg_line(const LineArgs<T> &a, T rden, const YzShape<T> &Y, bo
 // multipliers and a zero right-hand side), which carry nothing across.
 // Input and output strides differ for the staggered operators (n_in !=
 // n_out); byte offsets are 32-bit (the host checks both field sizes).
-template <typename T, int OP, int L, bool EPI, int LB>
-__global__ void __launch_bounds__(LB) seg_wall_kernel(const LineArgs<T> A, const int kdepth) {
+template <typename T, int OP, int L, bool EPI, int LB, int WPE = 1>
+__global__ void __launch_bounds__(LB, WPE) seg_wall_kernel(const LineArgs<T> A, const int kdepth) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *smem = reinterpret_cast<T *>(smem_raw);
   const int TL = blockDim.x, P = blockDim.y;
@@ -1014,9 +1014,12 @@ __global__ void __launch_bounds__(LB) seg_wall_kernel(const LineArgs<T> A, const
 }
 
 // Shape of seg_wall_kernel: 128-byte rows, segments of the precision's
-// segment length, doubled when the line needs more than 1024 threads (up
-// to 640 threads then, so that the longer windows keep a register budget of
-// 640 threads and stay out of scratch); longer lines go to line_kernel.
+// segment length; one point longer when the line needs more than 1024
+// threads with it (the 513-point wall-normal lines of a 512-cell channel:
+// 31 segments of 17 FP32 points, 57 of 9 FP64 points, with the short
+// windows' registers); doubled beyond that (up to 640 threads then, so
+// that the longer windows keep a register budget of 640 threads and stay
+// out of scratch); longer lines go to line_kernel.
 template <typename T>
 struct WallShape {
   static constexpr int L1 = sizeof(T) == 8 ? SEG_L64 : SEG_L32;
@@ -1028,7 +1031,10 @@ struct WallShape {
     L = L1;
     P = (n_out + L - 1) / L;
     if (TL * P > 1024) {
-      TL = 128 / (int)sizeof(T);
+      L = L1 + 1;
+      P = (n_out + L - 1) / L;
+    }
+    if (TL * P > 1024) {
       L = 2 * L1;
       P = (n_out + L - 1) / L;
       if (TL * P > 640) return false;
@@ -1042,11 +1048,17 @@ template <typename T, int OP>
 static int launch_seg_wall(const LineArgs<T> &a, const WallShape<T> &Y, bool epi, int kdepth, hipStream_t st) {
   const dim3 grid((a.na + Y.TL - 1) / Y.TL, a.nb), block(Y.TL, Y.P);
   constexpr int L1 = WallShape<T>::L1;
-  // the set's chain depth counts segments of L1 points
-  const int K = std::min(Y.P, Y.L == L1 ? kdepth : (kdepth + 1) / 2);
+  // the set's chain depth counts segments of L1 points (a segment of
+  // L1 + 1 points attenuates at least as much as one of L1)
+  const int K = std::min(Y.P, Y.L <= L1 + 1 ? kdepth : (kdepth + 1) / 2);
   if (Y.L == L1) {
     if (epi) hipLaunchKernelGGL((seg_wall_kernel<T, OP, L1, true, 1024>), grid, block, Y.shmem, st, a, K);
     else hipLaunchKernelGGL((seg_wall_kernel<T, OP, L1, false, 1024>), grid, block, Y.shmem, st, a, K);
+  } else if (Y.L == L1 + 1) {
+    // the plain operator fits two blocks of up to 16 waves per CU (at most
+    // 64 VGPRs); the fused-operand form would spill under that budget
+    if (epi) hipLaunchKernelGGL((seg_wall_kernel<T, OP, L1 + 1, true, 1024>), grid, block, Y.shmem, st, a, K);
+    else hipLaunchKernelGGL((seg_wall_kernel<T, OP, L1 + 1, false, 1024, 8>), grid, block, Y.shmem, st, a, K);
   } else {
     if (epi) hipLaunchKernelGGL((seg_wall_kernel<T, OP, 2 * L1, true, 640>), grid, block, Y.shmem, st, a, K);
     else hipLaunchKernelGGL((seg_wall_kernel<T, OP, 2 * L1, false, 640>), grid, block, Y.shmem, st, a, K);
