@@ -1120,14 +1120,27 @@ __global__ void __launch_bounds__(256) xwave_kernel(const LineArgs<T> A, int nli
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const int n = A.n_out;   // == n_in == 64*V
   const int e0 = lane * V;
+  // 512-point lines read their coefficients from LDS for every line (laid
+  // out [coefficient][point of the lane][lane], conflict-free) instead of
+  // holding 4V of them in registers
+  constexpr bool CL = V == 8;
+  __shared__ T ctab[CL ? 4 * 64 * V : 1];
   T fs[V], fw[V], ffw[V], sm[V];
+  if constexpr (CL) {
+    for (int i = threadIdx.x; i < 4 * n; i += blockDim.x) {
+      const int e = i >> 2, k = i & 3, ln = e / V, m = e - ln * V;
+      ctab[(k * V + m) * 64 + ln] = A.pk[i];
+    }
+    __syncthreads();
+  } else {
 #pragma unroll
-  for (int m = 0; m < V; ++m) {
-    const int e = e0 + m;
-    fs[m] = A.pk[4 * e + 0];
-    fw[m] = A.pk[4 * e + 1];
-    ffw[m] = A.pk[4 * e + 2];
-    sm[m] = A.pk[4 * e + 3];
+    for (int m = 0; m < V; ++m) {
+      const int e = e0 + m;
+      fs[m] = A.pk[4 * e + 0];
+      fw[m] = A.pk[4 * e + 1];
+      ffw[m] = A.pk[4 * e + 2];
+      sm[m] = A.pk[4 * e + 3];
+    }
   }
   // the next line's input is loaded while the current one is solved
   T nxt_u[V], nxt_u2[V];
@@ -1157,6 +1170,16 @@ __global__ void __launch_bounds__(256) xwave_kernel(const LineArgs<T> A, int nli
     T r[V];
 #pragma unroll
     for (int m = 0; m < V; ++m) r[m] = v[m];
+    if constexpr (CL) {
+      asm volatile("" ::: "memory");   // keep the table reads inside the loop
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        fs[m] = ctab[(0 * V + m) * 64 + lane];
+        fw[m] = ctab[(1 * V + m) * 64 + lane];
+        ffw[m] = ctab[(2 * V + m) * 64 + lane];
+        sm[m] = ctab[(3 * V + m) * 64 + lane];
+      }
+    }
     wave_solve<T, V>(r, fs, fw, ffw, sm, A.alfa, A.rden, lane);
 #pragma unroll
     for (int m = 0; m < V; ++m) v[m] = r[m];
