@@ -246,8 +246,13 @@ __device__ __forceinline__ void correct(const ProjArgs<T> &A, const int c, const
   }
 }
 
+// FP32 gradient sweep over 256-point lines: held to 128 VGPRs (4 waves per
+// SIMD instead of 3), which it fits without spilling
 template <typename T, int V, int SW>
-__global__ void __launch_bounds__(256) proj_x_kernel(const ProjArgs<T> A, int nlines) {
+constexpr int projx_wpe() { return (sizeof(T) == 4 && V == 4 && SW == SW_GRAD_X) ? 4 : 1; }
+
+template <typename T, int V, int SW>
+__global__ void __launch_bounds__(256, (projx_wpe<T, V, SW>())) proj_x_kernel(const ProjArgs<T> A, int nlines) {
   using R = Recipe<SW>;
   constexpr int OPI = R::PV ? OP_INTERPV : OP_INTERVP;
   constexpr int OPD = R::PV ? OP_DERPV : OP_DERVP;
