@@ -68,9 +68,18 @@ __device__ __forceinline__ T interior_row(const T (&v)[W], const int k, const St
     // derive.f90:34-36
     return s.afi * (v[k + 1] - v[k - 1]) + s.bfi * (v[k + 2] - v[k - 2]);
   } else if constexpr (OP == OP_DER2) {
-    // derive.f90:1414-1421
-    return s.asi * (v[k + 1] - v[k] - v[k] + v[k - 1]) + s.bsi * (v[k + 2] - v[k] - v[k] + v[k - 2]) +
-           s.csi * (v[k + 3] - v[k] - v[k] + v[k - 3]) + s.dsi * (v[k + 4] - v[k] - v[k] + v[k - 4]);
+    // derive.f90:1414-1421, sum_j c_j (U(i+j) - 2U(i) + U(i-j)), written on
+    // the second differences D(i) = (U(i+1) - U(i)) - (U(i) - U(i-1)):
+    // U(i+j) - 2U(i) + U(i-j) = sum_{|q|<j} (j - |q|) D(i+q), so the row is
+    // W0 D(i) + W1 (D(i-1) + D(i+1)) + W2 (..) + W3 (..) with W0 = c1 + 2c2 +
+    // 3c3 + 4c4, W1 = c2 + 2c3 + 3c4, W2 = c3 + 2c4, W3 = c4.  The first and
+    // second differences are shared by neighbouring rows of a window (9
+    // operations a row instead of 16), and the sums are of small numbers.
+    auto d1 = [&](int q) { return v[k + q + 1] - v[k + q]; };
+    auto d2 = [&](int q) { return d1(q) - d1(q - 1); };
+    const T w3 = s.dsi, w2 = s.csi + T(2) * s.dsi, w1 = s.bsi + T(2) * s.csi + T(3) * s.dsi,
+            w0 = s.asi + T(2) * s.bsi + T(3) * s.csi + T(4) * s.dsi;
+    return w0 * d2(0) + w1 * (d2(-1) + d2(1)) + w2 * (d2(-2) + d2(2)) + w3 * (d2(-3) + d2(3));
   } else if constexpr (OP == OP_DERVP) {
     // derive.f90:3826-3827
     return s.aci * (v[k + 1] - v[k]) + s.bci * (v[k + 2] - v[k - 1]);
