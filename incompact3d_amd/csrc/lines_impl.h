@@ -62,11 +62,18 @@ template <> struct Halo<OP_INTERPV> { static constexpr int L = 4, R = 3; };
 
 // ---------------------------------------------------------------- stencils
 // Interior rows: v[k] holds U(i) of output row i; offsets are compile-time.
-template <int OP, typename T, int W>
+// D2S selects the shared-second-difference form of the second derivative
+// (the direct four-term form needs fewer registers where a kernel is at its
+// register budget).
+template <int OP, typename T, int W, bool D2S = true>
 __device__ __forceinline__ T interior_row(const T (&v)[W], const int k, const Sten<T> &s) {
   if constexpr (OP == OP_DER1) {
     // derive.f90:34-36
     return s.afi * (v[k + 1] - v[k - 1]) + s.bfi * (v[k + 2] - v[k - 2]);
+  } else if constexpr (OP == OP_DER2 && !D2S) {
+    // derive.f90:1414-1421
+    return s.asi * (v[k + 1] - v[k] - v[k] + v[k - 1]) + s.bsi * (v[k + 2] - v[k] - v[k] + v[k - 2]) +
+           s.csi * (v[k + 3] - v[k] - v[k] + v[k - 3]) + s.dsi * (v[k + 4] - v[k] - v[k] + v[k - 4]);
   } else if constexpr (OP == OP_DER2) {
     // derive.f90:1414-1421, sum_j c_j (U(i+j) - 2U(i) + U(i-j)), written on
     // the second differences D(i) = (U(i+1) - U(i)) - (U(i) - U(i-1)):
@@ -976,6 +983,8 @@ __global__ void __launch_bounds__(LB, WPE) seg_wall_kernel(const LineArgs<T> A, 
   }
   for (int i = tid; i < 4 * np; i += nthr) cs[i] = i < 4 * n_out ? A.pk[i] : T(0);
   __syncthreads();
+  // the FP64 9-point segments with the 64-register budget: direct form
+  constexpr bool D2S = !(sizeof(T) == 8 && L == SEG_L64 + 1 && WPE > 1);
   T w[1][L];
 #pragma unroll
   for (int m = 0; m < L; ++m) {
@@ -983,7 +992,7 @@ __global__ void __launch_bounds__(LB, WPE) seg_wall_kernel(const LineArgs<T> A, 
     T r = T(0);
     if (e < n_out)
       r = is_boundary_row<OP>(e, n_out) ? bnd[line * 8 + (e < 4 ? e : e - (n_out - 8))]
-                                        : interior_row<OP, T, L + 8>(win, m + 4, A.s);
+                                        : interior_row<OP, T, L + 8, D2S>(win, m + 4, A.s);
     w[0][m] = r;
   }
   seg_solve<T, L, 1, 1, false>(w, e0, cs, A.alfa, T(0), cs, A.alfa, T(0), cf, cb, gf, gb, nullptr, nullptr, line,
