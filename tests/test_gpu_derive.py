@@ -1,7 +1,7 @@
 """GPU parity of the compact derivatives and staggered operators (derive.f90)
 against the CPU oracle, through the C-ABI mirror of the reference signatures.
 
-Bar: max|gpu - oracle| / max|oracle| <= 1e-12 in FP64,
+Bar: max|gpu - oracle| / max|oracle| <= 1e-12 in FP64 (the FP64 parity tolerance),
 <= 2e-5 for the FP32 entries (FP32 storage and arithmetic).
 
 The same checks with host (numpy) fields, through the x3d_host_* twins, are

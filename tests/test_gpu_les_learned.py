@@ -30,8 +30,9 @@ def test_learned_closure_gains_over_smagorinsky(torch_cuda):
     out = les.experiment(log=lambda m: print(m, flush=True))
     runs = out["runs"]
     print(json.dumps({k: v for k, v in out.items() if k not in ("history", "params")}, indent=1), flush=True)
-    dest = os.path.join(os.environ.get("GRAFT_REPO_ROOT", "."), "gpurun_out")
-    if os.path.isdir(dest):
+    # X3D_RESULTS_DIR, when set to a directory, receives the full record
+    dest = os.environ.get("X3D_RESULTS_DIR", "")
+    if dest and os.path.isdir(dest):
         with open(os.path.join(dest, "les_experiment.json"), "w") as f:
             json.dump({k: v for k, v in out.items() if k != "params"}, f)
     lr, sm, no = runs["learned"], runs["smagorinsky"], runs["none"]
