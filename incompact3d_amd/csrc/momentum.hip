@@ -77,6 +77,7 @@ __global__ void __launch_bounds__(1024) mom_yz_kernel(const MomArgs<T> A) {
   // matrix), w[2] of D2(u_c) (second-derivative matrix); u_c is read over
   // [e0-4, e0+L+4), u_d over [e0-2, e0+L+2), each once, before the
   // coefficient copy and its barrier
+  __builtin_amdgcn_s_setprio(3);   // issue this block's loads ahead of other blocks' arithmetic
   TableCopy<T, 2> tc;
   tc.load(A.pk1, A.pk2, tid, nthr, 4 * n);
   T w[3][L], ud[L];
@@ -101,6 +102,7 @@ __global__ void __launch_bounds__(1024) mom_yz_kernel(const MomArgs<T> A) {
       w[2][m] = interior_row<OP_DER2, T, L + 8>(wc, m + 4, A.s);
     }
   }
+  __builtin_amdgcn_s_setprio(0);   // back to normal priority once the loads are out
   tc.store(cs1, cs2, A.pk1, A.pk2, tid, nthr, 4 * n);
   __syncthreads();
   seg_solve<T, L, 3, 2>(w, e0, cs1, A.alfa1, A.rden1, cs2, A.alfa2, A.rden2, cf, cb, gf, gb, x0s, xns, line, seg,

@@ -111,6 +111,7 @@ __global__ void __launch_bounds__(1024) proj_yz_kernel(const ProjArgs<T> A) {
   const unsigned base = ((unsigned)al * (unsigned)A.sa + (unsigned)b * (unsigned)A.sb) * (unsigned)sizeof(T);
   const unsigned step = (unsigned)A.se * (unsigned)sizeof(T);
   auto wrap = [&](int j) { return (unsigned)(j < 0 ? j + n : (j >= n ? j - n : j)); };
+  __builtin_amdgcn_s_setprio(3);   // issue this block's loads ahead of other blocks' arithmetic
   TableCopy<T, 2> tc;
   tc.load(A.pkI, A.pkD, tid, nthr, 4 * n);
   // w[0..NI-1]: interpolations, w[NI]: derivative
@@ -136,6 +137,7 @@ __global__ void __launch_bounds__(1024) proj_yz_kernel(const ProjArgs<T> A) {
       for (int m = 0; m < L; ++m) w[N - 1][m] = interior_row<OPD, T, L + 8>(win, m + 4, A.s);
     }
   }
+  __builtin_amdgcn_s_setprio(0);   // back to normal priority once the loads are out
   tc.store(csI, csD, A.pkI, A.pkD, tid, nthr, 4 * n);
   __syncthreads();
   seg_solve<T, L, N, R::NI>(w, e0, csI, A.alfaI, A.rdenI, csD, A.alfaD, A.rdenD, cf, cb, gf, gb, x0s, xns, line,
